@@ -1,0 +1,198 @@
+/*
+ * massccs_amd -- C ABI of the MI355X trajectory-method engine.
+ *
+ * The engine is a drop-in for the hot path of MassCCS (cces-cepid/massccs):
+ * the Monte-Carlo trajectory loop that the reference runs inside
+ * System::System (src/System.cpp:160-317), its per-trajectory integrators
+ * System::run_He / run_N2 / run_CO2 (src/headers/System.h:92-94,
+ * src/System.cpp:338-1392), their setup (src/System.cpp:1397-1518) and the
+ * per-site force kernels Force::<kernel> (src/headers/Force.h:42-62).
+ * The host-side preparation that feeds them (src/Input.cpp, src/MoleculeTarget.cpp,
+ * src/GasBuffer.cpp, src/LinkedCell.cpp, System::geometric_ellipsoid) is
+ * restated here so that the same input.json / .xyz / .pqr / .ff files drive it.
+ *
+ * Conventions: plain pointers and sizes, no C++ or torch types.  Every call
+ * returns an mccs_status; on failure mccs_last_error() (thread-local) holds a
+ * message.  Units are the reference's: Angstrom, fs, amu, kcal/mol, e.
+ * Handles are owned by the caller and released with the matching *_destroy.
+ */
+#ifndef MASSCCS_H
+#define MASSCCS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MCCS_OK = 0,
+  MCCS_ERR_INVALID = 1,     /* bad argument or inconsistent input */
+  MCCS_ERR_IO = 2,          /* file missing or unreadable */
+  MCCS_ERR_PARSE = 3,       /* malformed json/xyz/pqr/ff */
+  MCCS_ERR_UNSUPPORTED = 4, /* option outside what the engine implements */
+  MCCS_ERR_DEVICE = 5,      /* HIP runtime error or no usable GPU */
+  MCCS_ERR_NOMEM = 6
+} mccs_status;
+
+/* Message of the last failed call on this thread. */
+const char *mccs_last_error(void);
+
+/* Probe gases, src/Input.cpp:98-117 ("GasBuffer"). */
+enum { MCCS_GAS_HE = 1, MCCS_GAS_N2 = 2, MCCS_GAS_CO2 = 3, MCCS_GAS_AR = 4, MCCS_GAS_CO2_1SITE = 5 };
+
+/* Run configuration: the fields of the reference's Input
+ * (src/headers/Input.h:49-67, defaults src/headers/Constants.h:68-83 and
+ * src/Input.cpp:75-262).  Strings are NUL-terminated paths. */
+typedef struct {
+  char target_file[1024];
+  char force_field[1024];   /* empty: built-in table (MoleculeTarget::defaultFF) */
+  uint32_t n_probe, n_iter, seed, nthreads;
+  double dt, temperature, skin, lj_cutoff, coul_cutoff, alpha;
+  int32_t gas;              /* MCCS_GAS_* */
+  int32_t equipotential;    /* must be 0: the equipotential ellipsoid is not built */
+  int32_t short_range_cutoff, long_range_flag, long_range_cutoff, polarizability;
+  int32_t user_ff;          /* 1 if force_field is set */
+  int32_t alpha_set;        /* 1 if alpha carries a value */
+} mccs_config;
+
+/* Reference defaults (Input::readInputFile with an input holding only
+ * targetFileName).  Replaces src/Input.cpp:49-263. */
+mccs_status mccs_config_default(mccs_config *cfg);
+
+/* Parse input.json with the reference's keys, defaults and error exits
+ * turned into status codes.  Relative file names in the json are kept as
+ * given (resolved against the process working directory, as the reference
+ * does).  Replaces Input::Input (src/Input.cpp:23-263). */
+mccs_status mccs_config_load_json(const char *path, mccs_config *cfg);
+
+/* ------------------------------------------------------------------ */
+/* Prepared system: target read, parameterised, centred and rotated onto its
+ * principal axes (MoleculeTarget, src/MoleculeTarget.cpp:19-931), probe
+ * constants (GasBuffer, src/GasBuffer.cpp:19-195), force type and reduced
+ * mass (src/System.cpp:49-117), geometric ellipsoid (src/System.cpp:1640-1698)
+ * and linked cells with atoms reordered by cell (src/LinkedCell.cpp:19-440). */
+typedef struct mccs_prepared mccs_prepared;
+
+mccs_status mccs_prepare(const mccs_config *cfg, mccs_prepared **out);
+
+/* Same preparation from atoms held in memory (synthetic targets): n atoms
+ * with element symbols (used against the force-field table), positions and
+ * charges; charges ignored for force types 1/2 as for a 5-column .xyz. */
+mccs_status mccs_prepare_atoms(const mccs_config *cfg, int64_t n,
+                               const char *const *symbols, const double *xyz,
+                               const double *q, mccs_prepared **out);
+
+void mccs_prepared_destroy(mccs_prepared *p);
+
+typedef struct {
+  int32_t gas, force_type, nsites, natoms;
+  int32_t nx, ny, nz, ncells;
+  int32_t filled_cells, max_atoms_cell;
+  double mass, charge, mu, alpha;       /* alpha in kcal/mol units */
+  double a, b, c, bmax;
+  double half_box[3];
+  double cell;                           /* lj_cutoff + skin */
+  double corner[3];
+  double radius;
+  int64_t stencil1_atoms, stencil2_atoms; /* atoms over all cells' stencils */
+} mccs_prepared_info;
+
+mccs_status mccs_prepared_get_info(const mccs_prepared *p, mccs_prepared_info *info);
+
+/* Sorted target arrays (cell order), each of length natoms; any pointer may
+ * be NULL.  eps_c/sig_c are the CO2 carbon-site pair parameters. */
+mccs_status mccs_prepared_get_atoms(const mccs_prepared *p, double *x, double *y,
+                                    double *z, double *q, double *m, double *eps,
+                                    double *sig, double *eps_c, double *sig_c);
+
+/* Text the reference prints from the start of the run up to and including
+ * "linked-cell calculation time" (Input::printReadInput,
+ * MoleculeTarget::printFF/print, src/System.cpp:91-144).  len receives the
+ * full length; at most cap-1 bytes are copied (NUL-terminated). */
+mccs_status mccs_prepared_report(const mccs_prepared *p, char *buf, size_t cap, size_t *len);
+
+/* Monte-Carlo samples of trajectories [first, first+n) in the reference's
+ * mt19937 stream order (src/System.cpp:184-197): 9 doubles per trajectory,
+ * u[0] = impact parameter b, u[1..4] = speed, orientation draws, u[5..8] the
+ * probe-orientation draws (N2/CO2; zero otherwise). */
+mccs_status mccs_samples(const mccs_prepared *p, uint64_t first, uint64_t n, double *u);
+
+/* ------------------------------------------------------------------ */
+/* Device engine on one GPU: the prepared target, cell index and speed table
+ * resident in HBM. */
+typedef struct mccs_engine mccs_engine;
+
+typedef struct {
+  int32_t device;          /* HIP device ordinal */
+  int32_t lanes;           /* lanes cooperating on one trajectory: 0 = auto, 16, 32 or 64 */
+  int32_t block;           /* threads per block: 0 = auto, else 256 or 512 */
+  int32_t use_lds;         /* -1 auto, 0 global-memory atoms, 1 atoms staged in LDS */
+  int32_t count_work;      /* 1: fill the work counters of mccs_traj_record */
+} mccs_engine_opts;
+
+mccs_status mccs_engine_create(const mccs_prepared *p, const mccs_engine_opts *opts,
+                               mccs_engine **out);
+void mccs_engine_destroy(mccs_engine *e);
+
+/* One trajectory's result.  hit/success/chi follow the reference loop body
+ * (src/System.cpp:206-230); tries is trajTries at exit, steps the integration
+ * steps over all tries; domega = pi (1 - cos chi) bmax^2 when scattered.
+ * The work counters (summed over every site evaluation of the trajectory) are
+ * filled when mccs_engine_opts.count_work is set: visits = (site, atom) pairs
+ * examined, lj = pairs inside the LJ cutoff, coul = Coulomb pairs, dip =
+ * induced-dipole pairs. */
+typedef struct {
+  int32_t hit, success, tries, steps;
+  double chi, domega;
+  uint64_t visits, lj, coul, dip;
+} mccs_traj_record;
+
+/* Runs trajectories over host-resident samples (n x 9 doubles, as written by
+ * mccs_samples) and writes host records.  Replaces the OpenMP loop of
+ * src/System.cpp:204-285 and the System::setup/run_* calls in it. */
+mccs_status mccs_run(mccs_engine *e, const double *samples, uint64_t n,
+                     mccs_traj_record *records);
+
+/* Same with device-resident samples and records (HBM pointers), enqueued on
+ * a HIP stream (NULL = the engine's stream) without synchronising. */
+mccs_status mccs_run_device(mccs_engine *e, const double *samples_dev, uint64_t n,
+                            mccs_traj_record *records_dev, void *stream);
+
+/* Site roles for mccs_eval_sites.  For N2: MCCS_SITE_N uses
+ * Force::lennardjones_coulomb[_LC] (or lennardjones[_LC] for types 1/2) with
+ * charge -0.4825; MCCS_SITE_DUMMY uses coulomb[_LC] or
+ * coulomb_induced_dipole_iso[_LC] with charge +0.965.  For He/Ar/co2:
+ * MCCS_SITE_ATOM uses lennardjones[_LC] / lennardjones_induced_dipole[_LC].
+ * For CO2: MCCS_SITE_O (lennardjones_coulomb*) and MCCS_SITE_C (the *_CO2
+ * kernels). */
+enum { MCCS_SITE_ATOM = 0, MCCS_SITE_N = 1, MCCS_SITE_DUMMY = 2, MCCS_SITE_O = 3, MCCS_SITE_C = 4 };
+
+/* Per-point site forces: for each of n host points (xyz, 3n doubles) the force
+ * (3n) and potential energy (n) of the probe site of the given role.
+ * Replaces Force::<kernel>(GasBuffer*, int iatom, vector<double>& f,
+ * double& Up) (src/headers/Force.h:42-62). */
+mccs_status mccs_eval_sites(mccs_engine *e, int32_t site, const double *xyz, uint64_t n,
+                            double *f, double *U);
+
+/* Tally over nIter x nTraj records in index order, src/System.cpp:287-317:
+ * omega_iter[i] = sum_j domega / (Nscatter + Nfree); counts[3i..3i+2] = Nfree,
+ * Nscatter, Nlost; mean and standard error of the nIter values. */
+mccs_status mccs_tally(const mccs_traj_record *records, uint64_t n_iter, uint64_t n_traj,
+                       double *omega_iter, int64_t *counts, double *ccs_mean,
+                       double *ccs_err);
+
+/* Same tally on the device over device-resident records; writes n_iter
+ * omega values, 3 n_iter counts and (mean, err) into device buffers. */
+mccs_status mccs_tally_device(mccs_engine *e, const mccs_traj_record *records_dev,
+                              uint64_t n_iter, uint64_t n_traj, double *omega_dev,
+                              int64_t *counts_dev, double *mean_err_dev, void *stream);
+
+/* Library version string. */
+const char *mccs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

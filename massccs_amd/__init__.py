@@ -1,0 +1,262 @@
+"""massccs_amd: MI355X trajectory-method engine for ion-neutral collision
+cross sections (a drop-in for the MassCCS trajectory loop).
+
+The product is libmassccs_amd.so (C ABI in include/massccs.h) and the
+`massccs` command line built next to this file.  This module is a thin ctypes
+view of that C ABI for Python callers (tests, benchmarks, scripts); it loads
+the in-tree library and raises if it has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmassccs_amd.so")
+CLI_PATH = os.path.join(HERE, "massccs")
+
+GAS = {"He": 1, "N2": 2, "CO2": 3, "Ar": 4, "co2": 5}
+SITE_ATOM, SITE_N, SITE_DUMMY, SITE_O, SITE_C = 0, 1, 2, 3, 4
+
+ERRORS = {1: "invalid argument", 2: "I/O error", 3: "parse error", 4: "unsupported",
+          5: "device error", 6: "out of memory"}
+
+
+class MassCCSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%s)" % (msg, ERRORS.get(code, code)))
+        self.code = code
+
+
+class Config(C.Structure):
+    """mccs_config of include/massccs.h (fields of the reference's Input)."""
+    _fields_ = [
+        ("target_file", C.c_char * 1024), ("force_field", C.c_char * 1024),
+        ("n_probe", C.c_uint32), ("n_iter", C.c_uint32), ("seed", C.c_uint32),
+        ("nthreads", C.c_uint32),
+        ("dt", C.c_double), ("temperature", C.c_double), ("skin", C.c_double),
+        ("lj_cutoff", C.c_double), ("coul_cutoff", C.c_double), ("alpha", C.c_double),
+        ("gas", C.c_int32), ("equipotential", C.c_int32), ("short_range_cutoff", C.c_int32),
+        ("long_range_flag", C.c_int32), ("long_range_cutoff", C.c_int32),
+        ("polarizability", C.c_int32), ("user_ff", C.c_int32), ("alpha_set", C.c_int32),
+    ]
+
+
+class PreparedInfo(C.Structure):
+    _fields_ = [
+        ("gas", C.c_int32), ("force_type", C.c_int32), ("nsites", C.c_int32),
+        ("natoms", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("ncells", C.c_int32), ("filled_cells", C.c_int32), ("max_atoms_cell", C.c_int32),
+        ("mass", C.c_double), ("charge", C.c_double), ("mu", C.c_double), ("alpha", C.c_double),
+        ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("bmax", C.c_double),
+        ("half_box", C.c_double * 3), ("cell", C.c_double), ("corner", C.c_double * 3),
+        ("radius", C.c_double), ("stencil1_atoms", C.c_int64), ("stencil2_atoms", C.c_int64),
+    ]
+
+
+class EngineOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("block", C.c_int32),
+                ("use_lds", C.c_int32), ("count_work", C.c_int32)]
+
+
+RECORD_DTYPE = np.dtype([("hit", "<i4"), ("success", "<i4"), ("tries", "<i4"), ("steps", "<i4"),
+                         ("chi", "<f8"), ("domega", "<f8"), ("visits", "<u8"), ("lj", "<u8"),
+                         ("coul", "<u8"), ("dip", "<u8")])
+
+# every symbol declared in include/massccs.h
+EXPORTS = [
+    "mccs_last_error", "mccs_config_default", "mccs_config_load_json", "mccs_prepare",
+    "mccs_prepare_atoms", "mccs_prepared_destroy", "mccs_prepared_get_info",
+    "mccs_prepared_get_atoms", "mccs_prepared_report", "mccs_samples", "mccs_engine_create",
+    "mccs_engine_destroy", "mccs_run", "mccs_run_device", "mccs_eval_sites", "mccs_tally",
+    "mccs_tally_device", "mccs_version",
+]
+
+_lib = None
+_dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lib():
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MassCCSError(5, "libmassccs_amd.so is not built (run __graft_entry__.build() or "
+                              "make -C massccs_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    L.mccs_last_error.restype = C.c_char_p
+    L.mccs_version.restype = C.c_char_p
+    st = C.c_int
+    L.mccs_config_default.argtypes = [C.POINTER(Config)]
+    L.mccs_config_load_json.argtypes = [C.c_char_p, C.POINTER(Config)]
+    L.mccs_prepare.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.mccs_prepare_atoms.argtypes = [C.POINTER(Config), C.c_int64, C.POINTER(C.c_char_p),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_void_p)]
+    L.mccs_prepared_destroy.argtypes = [C.c_void_p]
+    L.mccs_prepared_destroy.restype = None
+    L.mccs_prepared_get_info.argtypes = [C.c_void_p, C.POINTER(PreparedInfo)]
+    L.mccs_prepared_get_atoms.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 9
+    L.mccs_prepared_report.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.mccs_samples.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]
+    L.mccs_engine_create.argtypes = [C.c_void_p, C.POINTER(EngineOpts), C.POINTER(C.c_void_p)]
+    L.mccs_engine_destroy.argtypes = [C.c_void_p]
+    L.mccs_engine_destroy.restype = None
+    L.mccs_run.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.c_void_p]
+    L.mccs_run_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+    L.mccs_eval_sites.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mccs_tally.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
+                             C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mccs_tally_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]
+    for n in EXPORTS:
+        if n not in ("mccs_last_error", "mccs_version", "mccs_prepared_destroy",
+                     "mccs_engine_destroy"):
+            getattr(L, n).restype = st
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise MassCCSError(status, lib().mccs_last_error().decode(errors="replace"))
+
+
+def config_from_json(path):
+    c = Config()
+    _check(lib().mccs_config_load_json(os.fsencode(path), C.byref(c)))
+    return c
+
+
+def config_default(target_file=""):
+    c = Config()
+    _check(lib().mccs_config_default(C.byref(c)))
+    c.target_file = os.fsencode(target_file)
+    return c
+
+
+def config_from_dict(d):
+    """Config from a dict holding input.json keys (reference semantics)."""
+    import json
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as fh:
+        json.dump(d, fh)
+        name = fh.name
+    try:
+        return config_from_json(name)
+    finally:
+        os.unlink(name)
+
+
+class Prepared:
+    """Prepared target (MoleculeTarget + LinkedCell + ellipsoid) on the host."""
+
+    def __init__(self, cfg, symbols=None, xyz=None, q=None):
+        self.cfg = cfg
+        h = C.c_void_p()
+        if symbols is None:
+            _check(lib().mccs_prepare(C.byref(cfg), C.byref(h)))
+        else:
+            xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+            n = len(xyz)
+            arr = (C.c_char_p * n)(*[os.fsencode(s) for s in symbols])
+            qa = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
+            _check(lib().mccs_prepare_atoms(C.byref(cfg), n, arr, _dp(xyz),
+                                            None if qa is None else _dp(qa), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mccs_prepared_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def info(self):
+        i = PreparedInfo()
+        _check(lib().mccs_prepared_get_info(self.h, C.byref(i)))
+        return i
+
+    def atoms(self):
+        n = self.info.natoms
+        out = {k: np.zeros(n) for k in ("x", "y", "z", "q", "m", "eps", "sig", "eps_c", "sig_c")}
+        _check(lib().mccs_prepared_get_atoms(self.h, *[_dp(out[k]) for k in
+                                                       ("x", "y", "z", "q", "m", "eps", "sig",
+                                                        "eps_c", "sig_c")]))
+        return out
+
+    def report(self):
+        n = C.c_size_t()
+        _check(lib().mccs_prepared_report(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(lib().mccs_prepared_report(self.h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def samples(self, n, first=0):
+        u = np.zeros((n, 9))
+        _check(lib().mccs_samples(self.h, first, n, _dp(u)))
+        return u
+
+
+class Engine:
+    """Device engine on one GPU (target resident in HBM)."""
+
+    def __init__(self, prepared, device=0, lanes=0, block=0, use_lds=-1, count_work=True):
+        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0)
+        h = C.c_void_p()
+        _check(lib().mccs_engine_create(prepared.h, C.byref(o), C.byref(h)))
+        self.h, self.prepared = h, prepared
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mccs_engine_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def run(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        rec = np.zeros(len(u), dtype=RECORD_DTYPE)
+        _check(lib().mccs_run(self.h, _dp(u), len(u), rec.ctypes.data_as(C.c_void_p)))
+        return rec
+
+    def run_device(self, u_ptr, n, rec_ptr, stream=None):
+        _check(lib().mccs_run_device(self.h, C.c_void_p(u_ptr), n, C.c_void_p(rec_ptr),
+                                     C.c_void_p(stream) if stream else None))
+
+    def tally_device(self, rec_ptr, n_iter, n_traj, omega_ptr, counts_ptr, mean_err_ptr,
+                     stream=None):
+        _check(lib().mccs_tally_device(self.h, C.c_void_p(rec_ptr), n_iter, n_traj,
+                                       C.c_void_p(omega_ptr), C.c_void_p(counts_ptr),
+                                       C.c_void_p(mean_err_ptr),
+                                       C.c_void_p(stream) if stream else None))
+
+    def eval_sites(self, site, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        f = np.zeros_like(xyz)
+        U = np.zeros(len(xyz))
+        _check(lib().mccs_eval_sites(self.h, site, _dp(xyz), len(xyz), _dp(f), _dp(U)))
+        return f, U
+
+
+def tally(rec, n_iter, n_traj):
+    rec = np.ascontiguousarray(rec, dtype=RECORD_DTYPE)
+    om = np.zeros(n_iter)
+    cnt = np.zeros(3 * n_iter, dtype=np.int64)
+    mean, err = C.c_double(), C.c_double()
+    _check(lib().mccs_tally(rec.ctypes.data_as(C.c_void_p), n_iter, n_traj, _dp(om),
+                            cnt.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(mean),
+                            C.byref(err)))
+    return om, cnt.reshape(n_iter, 3), mean.value, err.value
+
+
+def build(verbose=False):
+    """Compile libmassccs_amd.so and the massccs CLI for gfx950 (in-tree)."""
+    import subprocess
+    r = subprocess.run(["make", "-C", os.path.join(HERE, "csrc"), "-j8"],
+                       capture_output=not verbose, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + (r.stdout or "") + (r.stderr or ""))

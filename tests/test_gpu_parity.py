@@ -1,0 +1,176 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle on
+the same prepared target and the same seeded samples.
+
+Criteria (per point): |dF_k| <= 1e-12 * sum_pairs |F_k,pair| and likewise for
+U (the reference's own fast-math and strict builds reorder these sums, so a
+plain relative error is not attainable where the total cancels).  Per
+trajectory: identical hit flags; for trajectories the oracle scatters with
+chi > 1e-6 rad and the same retry count, |dchi| <= 1e-8 rad unless the
+trajectory is chaotic (checked against the oracle at dt/2 is not possible per
+trajectory here, so a bounded fraction may exceed it); elsewhere
+|d(1 - cos chi)| <= 1e-15.
+"""
+import numpy as np
+import pytest
+
+from common import (LONG, M, O, C60_XYZ, UBQ_PQR, c60_lj_cfg, globule_cfg, gpu_available,
+                    oracle_for, pair_scales, prepared)
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+REL = 1e-12
+
+
+def _points(P, n, rng, spread=1.3):
+    i = P.info
+    h = np.array(i.half_box)
+    pts = rng.uniform(-spread, spread, size=(n, 3)) * np.array([i.a, i.b, i.c]) * 0.5
+    # a few points near atoms and a few outside the grid
+    a = P.atoms()
+    k = rng.integers(0, i.natoms, size=n // 4)
+    pts[: n // 4] = np.stack([a["x"][k], a["y"][k], a["z"][k]], 1) + rng.normal(0, 2.5, (n // 4, 3))
+    pts[-4:] = h * np.array([[1.02, 0, 0], [0, -1.01, 0.2], [0.3, 0.1, 1.05], [-1.5, -1.5, -1.5]])
+    return pts
+
+
+CASES = [
+    # name, cfg, site role, kernel name, probe charge, terms (lj, coul, dip), cutoff
+    ("c60_He_t4", dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", **LONG),
+     M.SITE_ATOM, "lennardjones_induced_dipole_LC", 0.0, (True, False, True), True),
+    ("ubq_He_t4", dict(targetFileName=UBQ_PQR, GasBuffer="He", polarizability="yes", **LONG),
+     M.SITE_ATOM, "lennardjones_induced_dipole_LC", 0.0, (True, False, True), True),
+    ("ubq_He_t2", dict(targetFileName=UBQ_PQR, GasBuffer="He", **{"Long-range forces": "no"}),
+     M.SITE_ATOM, "lennardjones_LC", 0.0, (True, False, False), True),
+    ("ubq_N2_t6_N", dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="yes", **LONG),
+     M.SITE_N, "lennardjones_coulomb_LC", -0.4825, (True, True, False), True),
+    ("ubq_N2_t6_D", dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="yes", **LONG),
+     M.SITE_DUMMY, "coulomb_induced_dipole_iso_LC", 0.965, (False, True, True), True),
+    ("ubq_N2_t3_N", dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="no",
+                         **{"Short-range cutoff": "no", "Long-range forces": "yes",
+                            "Long-range cutoff": "no"}),
+     M.SITE_N, "lennardjones_coulomb", -0.4825, (True, True, False), False),
+    ("ubq_CO2_t6_C", dict(targetFileName=UBQ_PQR, GasBuffer="CO2", polarizability="yes", **LONG),
+     M.SITE_C, "lennardjones_coulomb_induced_dipole_iso_LC_CO2", 0.6512, ("central", True, True),
+     True),
+    ("ubq_CO2_t6_O", dict(targetFileName=UBQ_PQR, GasBuffer="CO2", polarizability="yes", **LONG),
+     M.SITE_O, "lennardjones_coulomb_LC", -0.3256, (True, True, False), True),
+    ("ubq_Ar_t4", dict(targetFileName=UBQ_PQR, GasBuffer="Ar", polarizability="yes", **LONG),
+     M.SITE_ATOM, "lennardjones_induced_dipole_LC", 0.0, (True, False, True), True),
+]
+
+
+@pytest.mark.parametrize("name,cfg,site,kernel,qi,terms,cut", CASES, ids=[c[0] for c in CASES])
+def test_site_forces_match_oracle(name, cfg, site, kernel, qi, terms, cut):
+    cfg = dict(cfg, seed=2104)
+    print("case", name, flush=True)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    rng = np.random.default_rng(7)
+    pts = _points(P, 200, rng)
+    E = M.Engine(P)
+    f, U = E.eval_sites(site, pts)
+    worst = 0.0
+    for k in range(len(pts)):
+        fr, Ur = orc.site_force(kernel, pts[k], qi)
+        sf, su = pair_scales(P, cfg, pts[k], qi, terms[0], terms[1], terms[2], cut)
+        err = max(np.max(np.abs(f[k] - fr) / sf), abs(U[k] - Ur) / su)
+        worst = max(worst, err)
+        assert err <= REL, (name, k, pts[k], f[k], fr, U[k], Ur)
+    print(name, "worst relative error", worst)
+
+
+def _compare_traj(rec, ref, name):
+    hit_eq = np.array_equal(rec["hit"], ref["hit"])
+    assert hit_eq, name
+    both = (rec["success"] == 1) & (ref["success"] == 1) & (rec["tries"] == ref["tries"])
+    big = both & (ref["chi"] > 1e-6)
+    small = both & (ref["chi"] <= 1e-6)
+    dchi = np.abs(rec["chi"] - ref["chi"])
+    ok_big = dchi[big] <= 1e-8
+    ok_small = np.abs(np.cos(rec["chi"][small]) - np.cos(ref["chi"][small])) <= 1e-15
+    frac_flow = np.mean(rec["success"] == ref["success"])
+    print(name, "hits", rec["hit"].sum(), "scatter agree %.4f" % frac_flow,
+          "chi>1e-6: %d, within 1e-8: %d (max %.3g)" % (big.sum(), ok_big.sum(),
+                                                      dchi[big].max() if big.any() else 0),
+          "small chi ok %d/%d" % (ok_small.sum(), small.sum()))
+    return big, ok_big, ok_small, frac_flow
+
+
+@pytest.mark.parametrize("lanes", [16, 64])
+def test_trajectories_c60_he(lanes):
+    cfg = dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", seed=2104, **LONG)
+    print("case c60_He lanes", lanes, flush=True)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    u = P.samples(2000)
+    assert np.array_equal(u, orc.samples(2000))
+    ref = orc.run(u)
+    rec = M.Engine(P, lanes=lanes).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "c60_He")
+    assert flow >= 0.999
+    assert ok_big.mean() >= 0.99 and ok_small.all()
+
+
+def test_trajectories_ubq_n2():
+    cfg = dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="yes", seed=2104, **LONG)
+    print("case ubq_N2", flush=True)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    u = P.samples(200)
+    ref = orc.run(u)
+    rec = M.Engine(P).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "ubq_N2")
+    assert flow >= 0.98
+    assert ok_big.mean() >= 0.95 and ok_small.all()
+
+
+def test_trajectories_c60_lj(workdir):
+    cfg = c60_lj_cfg(workdir)
+    print("case c60_LJ", flush=True)
+    P = prepared(cfg)
+    assert P.info.force_type == 2
+    orc = oracle_for(P, cfg)
+    u = P.samples(3000)
+    ref = orc.run(u)
+    rec = M.Engine(P).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "c60_LJ")
+    assert flow >= 0.999
+    assert ok_big.mean() >= 0.99 and ok_small.all()
+
+
+def test_trajectories_globule_n2(workdir):
+    cfg = globule_cfg(workdir)
+    print("case globule_N2", flush=True)
+    P = prepared(cfg)
+    assert P.info.force_type == 6
+    orc = oracle_for(P, cfg)
+    u = P.samples(200)
+    ref = orc.run(u)
+    rec = M.Engine(P).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "globule_N2")
+    assert flow >= 0.98
+    assert ok_big.mean() >= 0.95 and ok_small.all()
+
+
+def test_tally_device_matches_host():
+    import torch
+    cfg = dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", seed=2104, **LONG)
+    P = prepared(cfg)
+    E = M.Engine(P)
+    u = P.samples(3000)
+    rec = E.run(u)
+    om, cnt, mean, err = M.tally(rec, 3, 1000)
+    om_o, cnt_o, mean_o, err_o = O.tally(rec[["hit", "success", "tries", "steps", "chi",
+                                              "domega"]].astype(O.RECORD_DTYPE), 3, 1000)
+    assert np.array_equal(om, om_o) and np.array_equal(cnt, cnt_o)
+    d_rec = torch.from_numpy(rec.view(np.uint8).copy()).cuda()
+    d_om = torch.zeros(3, dtype=torch.float64, device="cuda")
+    d_cnt = torch.zeros(9, dtype=torch.int64, device="cuda")
+    d_me = torch.zeros(2, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    E.tally_device(d_rec.data_ptr(), 3, 1000, d_om.data_ptr(), d_cnt.data_ptr(), d_me.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(d_om.cpu().numpy(), om)
+    assert np.array_equal(d_cnt.cpu().numpy().reshape(3, 3), cnt)
+    assert abs(d_me[0].item() - mean) <= 1e-12 * abs(mean)
