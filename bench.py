@@ -1,0 +1,202 @@
+"""Benchmark of the trajectory-method hot path on one MI355X.
+
+Workload: the 1,231-atom synthetic globule (ellipsoid 18/15/13 A, min spacing
+1 A, C:N:O:H:S = 0.31:0.08:0.10:0.50:0.01, net +4 e) with the N2 probe,
+LJ + Coulomb + ion-induced dipole with cutoffs 12/25 A (force type 6),
+10 x 10^4 trajectories, seed 2104, dt 10 fs, 298 K.
+
+One step = the whole Monte-Carlo CCS calculation on the GPU: every
+trajectory of the 10^5 samples (setup, velocity-Verlet/RATTLE, site forces)
+plus the tally, with the samples already resident in HBM.
+
+Printed: one JSON line with trajectories/s, the roofline of the trajectory
+kernel (FP64 work counted by the kernel itself, over its HIP-event duration)
+and the reference CPU program timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "trajectories/sec on 1 MI355X, plain vs surrogate+batched at ≤0.5% CCS error"
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations per unit of counted work (FMA = 2, sqrt = 1, div = 1):
+# every (site, atom) pair examined: difference vector, r^2, cutoff test (9);
+# LJ term (18); shifted Coulomb (19); smoothed induced dipole field and
+# gradient (39); the dipole epilogue per site evaluation (27).
+W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
+
+
+def workload_cfg(tmp):
+    from massccs_amd import synth
+    path = os.path.join(tmp, "globule_1231.xyz")
+    s, xyz, q = synth.globule(n=1231, axes=(18.0, 15.0, 13.0), net_charge=4.0, seed=2104)
+    synth.write_xyz(path, s, xyz, q, comment="synthetic globule 1231 atoms, net +4e")
+    return {"targetFileName": path, "GasBuffer": "N2", "Short-range cutoff": "yes",
+            "LJ-cutoff": 12.0, "Long-range forces": "yes", "Long-range cutoff": "yes",
+            "Coul-cutoff": 25.0, "polarizability": "yes", "alpha": 1.7403, "seed": 2104,
+            "dt": 10.0, "Temp": 298.0, "numberProbe": 10000, "nIter": 10}
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("flags"):
+                    return set(ln.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def cpu_baseline(cfg, budget_s=15.0):
+    """The reference program (oracle/_ref, built from the reference sources with
+    its release flags) on this host's cores, on a bounded sample of the same
+    workload: trajectories/s from its own "CCS time" stamp."""
+    ref_dir = os.path.join(HERE, "oracle", "_ref")
+    v4 = {"avx512f", "avx512bw", "avx512dq", "avx512vl", "avx512cd"} <= cpu_flags()
+    exe = os.path.join(ref_dir, "massccs_ref_v4" if v4 else "massccs_ref")
+    if not os.path.exists(exe):
+        return None
+    cores = host_cores()
+    tmp = tempfile.mkdtemp(prefix="mccs_cpu_")
+    try:
+        shutil.copy(cfg["targetFileName"], tmp)
+
+        def run(n):
+            c = dict(cfg, targetFileName=os.path.basename(cfg["targetFileName"]),
+                     numberProbe=int(n), nIter=1, nthreads=cores)
+            with open(os.path.join(tmp, "in.json"), "w") as fh:
+                json.dump(c, fh)
+            out = subprocess.run([exe, "in.json"], cwd=tmp, capture_output=True, text=True,
+                                 timeout=600).stdout
+            t = float(re.search(r"CCS time: ([0-9.eE+-]+) s", out).group(1))
+            return t
+        n0 = 64 * cores
+        t0 = run(n0)
+        n1 = int(min(100000, max(n0, n0 * budget_s / max(t0, 1e-3))))
+        t1 = run(n1)
+        return {"value": n1 / t1, "unit": "trajectories/s", "cores": cores, "kind": "reference",
+                "sample": "%d trajectories (first %d of the same seeded stream, nIter 1), "
+                          "reference CLI built %s -O3 -ffast-math -fopenmp, %d OpenMP threads, "
+                          "CCS time %.2f s" % (n1, n1, "-march=x86-64-v4" if v4 else
+                                               "-march=x86-64-v3", cores, t1)}
+    except Exception as e:  # the baseline is reported, never required
+        return {"value": None, "unit": "trajectories/s", "cores": cores, "kind": "reference",
+                "sample": "failed: %s" % e}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the trajectory engine is measured on a single "
+              "MI355X (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import massccs_amd as M
+
+    tmp = tempfile.mkdtemp(prefix="mccs_bench_")
+    cfg = workload_cfg(tmp)
+    P = M.Prepared(M.config_from_dict(cfg))
+    info = P.info
+    n_iter, n_traj = cfg["nIter"], cfg["numberProbe"]
+    n = n_iter * n_traj
+    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True)
+
+    u = torch.from_numpy(P.samples(n)).cuda()
+    rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    omega = torch.zeros(n_iter, dtype=torch.float64, device="cuda")
+    counts = torch.zeros(3 * n_iter, dtype=torch.int64, device="cuda")
+    mean_err = torch.zeros(2, dtype=torch.float64, device="cuda")
+    # a stream of our own (default priority): the kernels and the HIP events
+    # that time them are enqueued on it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+
+    def step():
+        E.run_device(u.data_ptr(), n, rec.data_ptr(), sptr)
+        E.tally_device(rec.data_ptr(), n_iter, n_traj, omega.data_ptr(), counts.data_ptr(),
+                       mean_err.data_ptr(), sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    k_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        E.run_device(u.data_ptr(), n, rec.data_ptr(), sptr)
+        e1.record(stream)
+        E.tally_device(rec.data_ptr(), n_iter, n_traj, omega.data_ptr(), counts.data_ptr(),
+                       mean_err.data_ptr(), sptr)
+        k_ms.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
+
+    r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    me = mean_err.cpu().numpy()
+    hits = r["hit"] == 1
+    evals = (1 + r["steps"] + np.maximum(r["tries"], 0)) * hits
+    flops = (W_VISIT * r["visits"].sum() + W_LJ * r["lj"].sum() + W_COUL * r["coul"].sum()
+             + W_DIP * r["dip"].sum() + W_EPI * evals.sum())
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    value = n * args.steps / wall
+    site_evals = 3 * int(r["steps"].sum())
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "trajectories/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded globule target and mt19937 samples)",
+        "config": {"workload": "plain FP64, 1231-atom globule + N2, LJ+Coulomb+induced dipole "
+                               "with cutoffs (force type 6), 10x10^4 trajectories",
+                   "n_atoms": info.natoms, "trajectories_per_step": n, "probe": "N2",
+                   "force_type": info.force_type, "lanes_per_trajectory": args.lanes or "auto",
+                   "parallelism": "1 GPU"},
+        "force_evals_per_s": round(site_evals / (kern_ms * 1e-3), 1),
+        "ccs": {"mean": float(me[0]), "err": float(me[1])},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "traj_kernel<N2, type 6>",
+                     "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    shutil.rmtree(tmp, ignore_errors=True)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

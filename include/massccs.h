@@ -127,7 +127,7 @@ typedef struct mccs_engine mccs_engine;
 typedef struct {
   int32_t device;          /* HIP device ordinal */
   int32_t lanes;           /* lanes cooperating on one trajectory: 0 = auto, 16, 32 or 64 */
-  int32_t block;           /* threads per block: 0 = auto, else 256 or 512 */
+  int32_t block;           /* threads per block: 0 = auto, else 128 or 256 */
   int32_t use_lds;         /* -1 auto, 0 global-memory atoms, 1 atoms staged in LDS */
   int32_t count_work;      /* 1: fill the work counters of mccs_traj_record */
 } mccs_engine_opts;

@@ -12,8 +12,7 @@ __global__ __launch_bounds__(512) void eval_kernel(Tgt T, const double *__restri
                                                    double *__restrict__ uout) {
   extern __shared__ __align__(16) char smem[];
   Shm sh;
-  stage(T, sh, smem, LDS);
-  group_scratch<L>(sh, smem, T, LDS);
+  stage<L, LDS, any_central<S>()>(T, sh, smem);
   const int lane = threadIdx.x % L;
   const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / L);
   Work w{0, 0, 0, 0};
@@ -21,7 +20,7 @@ __global__ __launch_bounds__(512) void eval_kernel(Tgt T, const double *__restri
     double p[1][3] = {{xyz[3 * g], xyz[3 * g + 1], xyz[3 * g + 2]}};
     double qs[1] = {qsite};
     double f[1][3], U[1];
-    evaluate<S, L, LDS, false>(T, sh, lane, p, qs, f, U, w);
+    evaluate<S, L, LDS, false>(T, sh, lane, p, qs, f, U, w, nullptr);
     if (lane == 0) {
       fout[3 * g] = f[0][0];
       fout[3 * g + 1] = f[0][1];
@@ -164,37 +163,14 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   if (E->gk == 1 && E->ft > 4) E->ft = 2;
   E->count = o.count_work != 0;
 
-  // LJ rows: distinct (eps, sig, eps_c, sig_c) tuples
-  std::vector<double> rows;
-  std::vector<uint8_t> type(P.natoms);
-  std::map<std::tuple<double, double, double, double>, int> ids;
-  const double rc6inv = 1.0 / std::pow(P.cfg.lj_cutoff, 6);
-  auto ljrow = [&](double eps, double sig, double *r) {
-    // src/Force.cpp:84-96 order: lj1 = 4 eps sig^6, lj2 = lj1 sig^6
-    const double lj1 = 4.0 * eps * std::pow(sig, 6.0);
-    const double lj2 = lj1 * std::pow(sig, 6.0);
-    r[0] = lj1;
-    r[1] = lj2;
-    r[2] = 6.0 * lj1;
-    r[3] = 12.0 * lj2;
-    r[4] = rc6inv * (lj2 * rc6inv - lj1);
-  };
+  // per-atom LJ pairs in the reference's order (src/Force.cpp:84-85):
+  // lj1 = 4 eps sig^6, lj2 = lj1 sig^6; the second pair for the CO2 carbon site
+  std::vector<double2> ljp(P.natoms), ljc(P.natoms);
   for (int64_t i = 0; i < P.natoms; i++) {
-    auto key = std::make_tuple(P.eps[i], P.sig[i], P.eps_c[i], P.sig_c[i]);
-    auto it = ids.find(key);
-    int id;
-    if (it == ids.end()) {
-      id = (int)ids.size();
-      if (id > 255) throw Error(MCCS_ERR_UNSUPPORTED, "more than 256 distinct LJ parameter sets");
-      ids[key] = id;
-      double r[10];
-      ljrow(P.eps[i], P.sig[i], r);
-      ljrow(P.eps_c[i], P.sig_c[i], r + 5);
-      rows.insert(rows.end(), r, r + 10);
-    } else {
-      id = it->second;
-    }
-    type[i] = (uint8_t)id;
+    const double l1 = 4.0 * P.eps[i] * std::pow(P.sig[i], 6.0);
+    ljp[i] = make_double2(l1, l1 * std::pow(P.sig[i], 6.0));
+    const double c1 = 4.0 * P.eps_c[i] * std::pow(P.sig_c[i], 6.0);
+    ljc[i] = make_double2(c1, c1 * std::pow(P.sig_c[i], 6.0));
   }
   std::vector<double4> atoms(P.natoms);
   for (int64_t i = 0; i < P.natoms; i++) atoms[i] = make_double4(P.x[i], P.y[i], P.z[i], P.q[i]);
@@ -205,14 +181,13 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
 
   Tgt &T = E->T;
   T.atoms = E->upload(atoms);
-  T.type = E->upload(type);
-  T.lj = E->upload(rows);
+  T.ljp = E->upload(ljp);
+  T.ljc = E->upload(ljc);
   T.cstart = E->upload(P.cell_start);
   T.vs = E->upload(P.vtab_s);
   T.vv = E->upload(P.vtab_v);
   T.stencil = E->upload(st);
   T.natoms = (int)P.natoms;
-  T.ntypes = (int)ids.size();
   T.nx = P.nx; T.ny = P.ny; T.nz = P.nz; T.ncells = P.ncells;
   T.vn = (int)P.vtab_s.size();
   T.n1 = (int)s1.size() / 3;
@@ -226,6 +201,7 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   T.rlj2lo = rlj * rlj * (1.0 - 1e-15); T.rlj2hi = rlj * rlj * (1.0 + 1e-15);
   T.rc2lo = rc * rc * (1.0 - 1e-15); T.rc2hi = rc * rc * (1.0 + 1e-15);
   T.irc3 = 1.0 / (rc * rc * rc);
+  T.rc6inv = 1.0 / std::pow(rlj, 6);
   T.cc1 = -1.5 / rc;
   T.cc2 = 0.5 / (rc * rc * rc);
   T.alpha = P.alpha;
@@ -234,6 +210,25 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   T.temp = P.cfg.temperature;
   T.ginertia = P.gas_inertia;
   T.d2 = P.d_bond * P.d_bond;
+  T.inv_cell = 1.0 / P.cell;
+  for (int d = 0; d < 3; d++) {
+    T.blo[d] = 1e300;
+    T.bhi[d] = -1e300;
+  }
+  for (int64_t i = 0; i < P.natoms; i++) {
+    const double v[3] = {P.x[i], P.y[i], P.z[i]};
+    for (int d = 0; d < 3; d++) {
+      T.blo[d] = std::min(T.blo[d], v[d]);
+      T.bhi[d] = std::max(T.bhi[d], v[d]);
+    }
+  }
+  {
+    const int ft = P.force_type, gk = (gas == MCCS_GAS_N2) ? 2 : (gas == MCCS_GAS_CO2 ? 3 : 1);
+    const bool longr = gk == 1 ? (ft == 3 || ft == 4) : ft >= 3;
+    const double rf = std::max(longr ? rc : rlj, rlj);
+    T.rfar2 = rf * rf * (1.0 + 1e-12);
+    T.rlong = longr ? rc : rlj;
+  }
   E->queue = reinterpret_cast<unsigned long long *>(E->dalloc(sizeof(unsigned long long)));
 
   // launch shape: lanes per trajectory by target size unless given
@@ -242,14 +237,23 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   if (L != 16 && L != 32 && L != 64) throw Error(MCCS_ERR_INVALID, "lanes must be 0, 16, 32 or 64");
   E->lanes = L;
   E->block = o.block ? o.block : 256;
-  if (E->block != 256 && E->block != 512) throw Error(MCCS_ERR_INVALID, "block must be 0, 256 or 512");
-  const size_t fixed = sizeof(double) * 10 * T.ntypes + sizeof(int) * 2 * E->block;
-  const size_t atom_bytes = (sizeof(double4) + 1) * (size_t)T.natoms;
-  const size_t lds_cap = 60 * 1024;
-  E->lds = o.use_lds < 0 ? (fixed + atom_bytes + 64 <= lds_cap) : (o.use_lds != 0);
-  if (E->lds && fixed + atom_bytes + 64 > 160 * 1024)
-    throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
-  E->shmem = fixed + (E->lds ? atom_bytes : 0) + 64;
+  if (E->block != 128 && E->block != 256) throw Error(MCCS_ERR_INVALID, "block must be 0, 128 or 256");
+  // Atoms (and their LJ pairs) are staged in LDS when the block's share fits
+  // 80 KB (two blocks per CU); Verlet lists then hold 16-bit indices.
+  const bool central = E->gk == 3;
+  const size_t per_block = 80 * 1024;
+  const bool fits = T.natoms <= 65535 &&
+                    lds_bytes(T.natoms, T.natoms, E->block, L, true, central) <= per_block;
+  E->lds = o.use_lds < 0 ? fits : (o.use_lds != 0);
+  if (E->lds && T.natoms > 65535) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
+  // list capacity: every atom for small targets, else what 120 KB of lists allow
+  const int groups = E->block / L;
+  const size_t lidx = E->lds ? 2 : 4;
+  const size_t base = lds_bytes(T.natoms, 0, E->block, L, E->lds, central);
+  const size_t room = base < 150 * 1024 ? 150 * 1024 - base : 0;
+  T.list_cap = (int)std::min<size_t>((size_t)T.natoms, room / (lidx * groups));
+  E->shmem = lds_bytes(T.natoms, T.list_cap, E->block, L, E->lds, central);
+  if (E->shmem > 160 * 1024) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
   return E.release();
 }
 
@@ -342,7 +346,7 @@ void engine_eval_sites(Engine *E, int site, const double *xyz, uint64_t n, doubl
   EvalFn fn = select_eval(lj, cen, coul, dip, cut, E->lds);
   if (!fn) throw Error(MCCS_ERR_UNSUPPORTED, "no site kernel for this role");
   const int block = 256;
-  const size_t shmem = E->shmem - sizeof(int) * 2 * E->block + sizeof(int) * 2 * block;
+  const size_t shmem = lds_bytes(E->T.natoms, E->T.list_cap, block, 64, E->lds, E->gk == 3);
   set_shmem((const void *)fn, shmem);
   double *dx = nullptr, *df = nullptr, *du = nullptr;
   HIP_OK(hipMalloc(&dx, sizeof(double) * 3 * n));

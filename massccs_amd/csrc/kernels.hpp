@@ -1,24 +1,24 @@
-#pragma once
 // MI355X (gfx950) trajectory engine: device code shared by the kernel
 // instantiation units (traj_gk*.hip) and the launcher (engine.hip).
 //
 // Work decomposition.  One trajectory is owned by a group of L lanes
 // (L = 16, 32 or 64) of one wavefront.  The group integrates the probe with
-// wave-uniform FP64 state, and its L lanes cooperate on every force
-// evaluation: the candidate atoms around the probe are flattened into one
-// index space and strided over the lanes, each lane accumulating all site
-// terms in registers; a DPP butterfly then leaves the identical sum in every
-// lane.  Blocks are persistent and pull trajectories from a device counter,
-// so trajectories of very different length share the GPU evenly.
+// group-uniform FP64 state kept in LDS, and its L lanes cooperate on every
+// force evaluation: the candidate atoms of the probe's Verlet list are strided
+// over the lanes, each lane accumulating every site term in registers; a DPP
+// butterfly (and scalar readlanes for 64-lane groups) then leaves the same sum
+// in every lane.  Blocks are persistent and pull trajectories from a device
+// counter, so trajectories of very different length share the GPU evenly.
 //
 // Reference behaviour restated here (cces-cepid/massccs):
-//   setup                   src/System.cpp:1397-1518
+//   setup                     src/System.cpp:1397-1518
 //   run_He / run_N2 / run_CO2 src/System.cpp:338-1392
-//   RATTLE                  src/System.cpp:1700-1767
-//   site kernels            src/Force.cpp (LJ :34-111, dipole :305-489,
-//                           Coulomb :598-723, dummy :903-1081, CO2 :1088-1725)
-//   cell index / stencils   src/LinkedCell.cpp:117-129, :228-440
-//   tally                   src/System.cpp:287-317
+//   RATTLE                    src/System.cpp:1700-1767
+//   site kernels              src/Force.cpp (LJ :34-111, dipole :305-489,
+//                             Coulomb :598-723, dummy :903-1081, CO2 :1088-1725)
+//   cell index / stencils     src/LinkedCell.cpp:117-129, :228-440
+//   tally                     src/System.cpp:287-317
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,15 +28,16 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
 #include "massccs.h"
 
-#define HIP_OK(x)                                                                        \
-  do {                                                                                   \
-    hipError_t e_ = (x);                                                                 \
-    if (e_ != hipSuccess)                                                                \
+#define HIP_OK(x)                                                                             \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess)                                                                     \
       throw ::mccs::Error(MCCS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
@@ -49,24 +50,31 @@ constexpr double KC = 332.06348078020324;                    // KCOUL
 constexpr double PI = 3.14159265358979323846;
 constexpr double FAR = 1.0e30;                               // parks an inactive site
 constexpr int MAX_STEPS = 4000000;                           // per trajectory, over all tries
+constexpr double SKIN = 1.0;                                 // Verlet-list skin (A)
 
 enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4 };
 
 // Kernel argument: target, grid and run constants.
 struct Tgt {
   const double4 *atoms;     // x, y, z, q in cell order
-  const uint8_t *type;      // LJ parameter row per atom
-  const double *lj;         // per row: lj1 lj2 lj3 lj4 ucut | same for the CO2 carbon site
+  const double2 *ljp;       // lj1 = 4 eps sig^6, lj2 = lj1 sig^6 per atom
+  const double2 *ljc;       // the same with the CO2 carbon-site parameters
   const int *cstart;        // ncells + 1 prefix of atoms per cell
   const double *vs, *vv;    // speed table: cumulative S_n and speeds v_n
   const int *stencil;       // (n1 + n2) * 3 offsets, first shell then second
-  int natoms, ntypes, nx, ny, nz, ncells, vn, n1, n2;
+  int natoms, nx, ny, nz, ncells, vn, n1, n2;
+  int list_cap;             // Verlet-list capacity per group
   double cell, cx, cy, cz;  // cell edge and grid corner
   double hx, hy, hz;        // half box
   double ea, eb, ec, bmax;  // ellipsoid semi-axes, maximal impact parameter
   double rlj, rc, rlj2lo, rlj2hi, rc2lo, rc2hi;
   double irc3, cc1, cc2;    // 1/rc^3, -1.5/rc, 0.5/rc^3
+  double rc6inv;            // 1/rlj^6 (LJ energy shift)
   double alpha, mu, dt, temp, ginertia, d2;
+  double inv_cell;          // 1 / cell (fast-path cell arithmetic only)
+  double blo[3], bhi[3];    // bounding box of the target atoms
+  double rfar2;             // no term reaches beyond this squared distance
+  double rlong;             // longest cutoff of the run's terms
 };
 
 // ---------------------------------------------------------------- probes
@@ -94,6 +102,8 @@ struct Spec {
     return GK == 2 ? (s == 0 ? 0.5488 : (s == 1 ? -0.5488 : 0.0))
                    : (GK == 3 ? (s == 0 ? -1.149 : (s == 1 ? 0.0 : 1.149)) : 0.0);
   }
+  // the site the others are symmetric about (N2 dummy, CO2 carbon)
+  static constexpr int centre() { return GK == 2 ? 2 : (GK == 3 ? 1 : 0); }
 };
 
 // One site of a given role, for per-point evaluation.
@@ -105,12 +115,20 @@ struct OneSite {
   static constexpr bool cen(int) { return CEN; }
   static constexpr bool coul(int) { return COUL; }
   static constexpr bool dip(int) { return DIP; }
+  static constexpr int centre() { return 0; }
 };
 
 template <class S>
 __device__ constexpr bool any_long() {
   bool r = false;
   for (int s = 0; s < S::NS; s++) r = r || S::coul(s) || S::dip(s);
+  return r;
+}
+
+template <class S>
+__device__ constexpr bool any_central() {
+  bool r = false;
+  for (int s = 0; s < S::NS; s++) r = r || (S::lj(s) && S::cen(s));
   return r;
 }
 
@@ -128,6 +146,11 @@ __device__ __forceinline__ double swz16(double v) {  // lane ^ 16 inside 32 lane
   hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
   return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
 
 // Butterfly sum over the L lanes of a group; every lane receives the same
 // bits (each step adds the same two operands in every lane).
@@ -136,9 +159,10 @@ __device__ __forceinline__ double gsum(double v) {
   v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
   v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
   v += dpp<0x141>(v);  // row_half_mirror
-  v += dpp<0x140>(v);  // row_mirror
-  if (L >= 32) v += swz16(v);
-  if (L >= 64) v += __shfl_xor(v, 32);
+  v += dpp<0x140>(v);  // row_mirror: every lane of a row holds the row sum
+  if (L == 32) v += swz16(v);
+  if (L == 64)  // the four row sums through scalar registers: wave-uniform by construction
+    v = (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
   return v;
 }
 
@@ -159,6 +183,23 @@ __device__ __forceinline__ int gscan_incl(int v, int lane) {
   return v;
 }
 
+// Bits of a wave ballot that belong to this lane's group, and the count of
+// set bits below this lane.
+template <int L>
+__device__ __forceinline__ void group_ballot(bool flag, int lane, int &below, int &total) {
+  const unsigned long long m = __ballot(flag);
+  const int gbase = (int)(threadIdx.x % 64) - lane;
+  const unsigned long long gm = (L == 64) ? m : ((m >> gbase) & ((1ull << L) - 1ull));
+  total = __popcll(gm);
+  below = __popcll(gm & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------- forces
 struct Acc {
   double U, fx, fy, fz, Ex, Ey, Ez, Exx, Eyy, Ezz, Exy, Exz, Eyz;
@@ -173,22 +214,28 @@ struct Work {
   uint32_t visits, lj, coul, dip;
 };
 
+struct GState;
+
 struct Shm {
-  int *incl;            // per-lane inclusive run ends of the group
-  int *base;            // per-lane (run start - run offset) * 2 + first-shell flag
-  const double *lj;     // LJ rows
-  const double4 *atoms; // LDS copy when staged
-  const uint8_t *type;
+  GState *state;          // the group's trajectory state
+  void *list;             // the group's Verlet list (uint16 indices when atoms are staged, else int)
+  int *incl;              // per-lane inclusive run ends of the group
+  int *base;              // per-lane (run start - run offset) * 2 + first-shell flag
+  const double4 *atoms;   // LDS copies when staged
+  const double2 *ljp, *ljc;
 };
 
-template <bool LDS>
-__device__ __forceinline__ void fetch(const Tgt &T, const Shm &sh, int a, double4 &at, int &ty) {
+template <bool LDS, bool CEN>
+__device__ __forceinline__ void fetch(const Tgt &T, const Shm &sh, int a, double4 &at, double2 &lp,
+                                      double2 &lc) {
   if (LDS) {
     at = sh.atoms[a];
-    ty = sh.type[a];
+    lp = sh.ljp[a];
+    if (CEN) lc = sh.ljc[a];
   } else {
     at = T.atoms[a];
-    ty = T.type[a];
+    lp = T.ljp[a];
+    if (CEN) lc = T.ljc[a];
   }
 }
 
@@ -196,15 +243,17 @@ __device__ __forceinline__ void fetch(const Tgt &T, const Shm &sh, int a, double
 // second-shell cells of an out-of-grid site (the reference applies LJ on the
 // first shell only).
 template <class S, bool COUNT>
-__device__ __forceinline__ void pair_terms(const Tgt &T, const Shm &sh, const double (&p)[S::NS][3],
-                                           const double (&qs)[S::NS], const double4 at, int ty,
-                                           int allow_lj, Acc (&a)[S::NS], Work &w) {
+__device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS][3],
+                                           const double (&qs)[S::NS], const double4 at,
+                                           const double2 lp, const double2 lc, int allow_lj,
+                                           Acc (&a)[S::NS], Work &w) {
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
     if (!(S::lj(s) || S::coul(s) || S::dip(s))) continue;
     const double dx = p[s][0] - at.x, dy = p[s][1] - at.y, dz = p[s][2] - at.z;
     const double r2 = dx * dx + dy * dy + dz * dz;
     if (COUNT) w.visits++;
+    const double lj1 = S::cen(s) ? lc.x : lp.x, lj2 = S::cen(s) ? lc.y : lp.y;
     if (S::CUT) {
       // r < cutoff as the reference tests it (sqrt(r2) < rc), decided on r2
       // except inside a relative band of 1e-15 around the cutoff
@@ -216,10 +265,10 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const Shm &sh, const do
         const double rinv = rsqrt(r2);
         const double r2inv = rinv * rinv;
         if (S::lj(s) && in_lj) {
-          const double *row = sh.lj + ty * 10 + (S::cen(s) ? 5 : 0);
           const double r6 = r2inv * r2inv * r2inv;
-          a[s].U += r6 * (row[1] * r6 - row[0]) - row[4];
-          const double flj = r6 * (row[3] * r6 - row[2]) * r2inv;
+          const double ucut = T.rc6inv * (lj2 * T.rc6inv - lj1);
+          a[s].U += r6 * (lj2 * r6 - lj1) - ucut;
+          const double flj = r6 * (12.0 * lj2 * r6 - 6.0 * lj1) * r2inv;
           a[s].fx += flj * dx;
           a[s].fy += flj * dy;
           a[s].fz += flj * dz;
@@ -259,10 +308,9 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const Shm &sh, const do
       const double rinv = rsqrt(r2);
       const double r2inv = rinv * rinv;
       if (S::lj(s)) {
-        const double *row = sh.lj + ty * 10 + (S::cen(s) ? 5 : 0);
         const double r6 = r2inv * r2inv * r2inv;
-        a[s].U += r6 * (row[1] * r6 - row[0]);
-        const double flj = r6 * (row[3] * r6 - row[2]) * r2inv;
+        a[s].U += r6 * (lj2 * r6 - lj1);
+        const double flj = r6 * (12.0 * lj2 * r6 - 6.0 * lj1) * r2inv;
         a[s].fx += flj * dx;
         a[s].fy += flj * dy;
         a[s].fz += flj * dz;
@@ -298,12 +346,12 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const Shm &sh, const do
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Candidate runs.  Mode 0 (probe inside the grid): one run per cell column
-// (i, j) that the sphere of radius R around the centre reaches, covering the
-// z-range of cells it cuts; atoms are stored in cell order with k fastest, so
-// a column range is contiguous.  Mode 1 (site outside the grid): the
-// reference stencils of the cell the reference's flat index aliases to.
-// Mode 2: all atoms (no-cutoff force types).
+// Candidate runs.  Mode 0: one run per cell column (i, j) that the sphere of
+// radius sqrt(R2) around c reaches, covering the z-range of cells it cuts;
+// atoms are stored in cell order with k fastest, so a column range is
+// contiguous.  Mode 1 (site outside the grid): the reference stencils of the
+// cell the reference's flat index aliases to.  Mode 2: all atoms
+// (no-cutoff force types).
 struct RunGen {
   int mode;
   double c[3], R2;
@@ -321,8 +369,8 @@ struct RunGen {
       const double d2 = dxm * dxm + dym * dym;
       if (d2 >= R2) return;
       const double h = sqrt(R2 - d2);
-      const int k0 = clampi((int)floor((c[2] - h - T.cz) / T.cell), 0, T.nz - 1);
-      const int k1 = clampi((int)floor((c[2] + h - T.cz) / T.cell), 0, T.nz - 1);
+      const int k0 = clampi((int)floor((c[2] - h - T.cz) * T.inv_cell), 0, T.nz - 1);
+      const int k1 = clampi((int)floor((c[2] + h - T.cz) * T.inv_cell), 0, T.nz - 1);
       const int b = T.nz * (j + T.ny * i);
       start = T.cstart[b + k0];
       len = T.cstart[b + k1 + 1] - start;
@@ -341,11 +389,11 @@ struct RunGen {
   }
 };
 
-// Flattens the runs of one pass over the group's lanes and accumulates.
-template <class S, int L, bool LDS, bool COUNT>
-__device__ void sweep(const Tgt &T, const Shm &sh, int lane, const RunGen &g, int nruns,
-                      const double (&p)[S::NS][3], const double (&qs)[S::NS], Acc (&a)[S::NS],
-                      Work &w) {
+// Walks the runs of one pass, flattening them over the group's lanes, and
+// hands every candidate atom index (with its first-shell flag) to visit().
+template <int L, class Visit>
+__device__ void walk_runs(const Tgt &T, const Shm &sh, int lane, const RunGen &g, int nruns,
+                          Visit &&visit) {
   for (int r0 = 0; r0 < nruns; r0 += L) {
     int start = 0, len = 0, first = 1;
     if (r0 + lane < nruns) g.get(T, r0 + lane, start, len, first);
@@ -353,18 +401,14 @@ __device__ void sweep(const Tgt &T, const Shm &sh, int lane, const RunGen &g, in
     const int total = __shfl(incl, L - 1, L);
     sh.incl[lane] = incl;
     sh.base[lane] = (start - (incl - len)) * 2 + first;
-    __builtin_amdgcn_wave_barrier();
+    lds_fence();
     int cur = 0;
     for (int t = lane; t < total; t += L) {
       while (cur < L - 1 && t >= sh.incl[cur]) cur++;
       const int bw = sh.base[cur];
-      const int at_i = (bw >> 1) + t;
-      double4 at;
-      int ty;
-      fetch<LDS>(T, sh, at_i, at, ty);
-      pair_terms<S, COUNT>(T, sh, p, qs, at, ty, bw & 1, a, w);
+      visit((bw >> 1) + t, bw & 1);
     }
-    __builtin_amdgcn_wave_barrier();
+    lds_fence();
   }
 }
 
@@ -392,107 +436,224 @@ __device__ __forceinline__ void reduce(Acc (&a)[S::NS]) {
   }
 }
 
+template <class S>
+__device__ __forceinline__ void finish(const Tgt &T, const Acc (&a)[S::NS], int only,
+                                       double (&f)[S::NS][3], double (&U)[S::NS]) {
+#pragma unroll
+  for (int s = 0; s < S::NS; s++) {
+    if (only >= 0 && s != only) continue;
+    if (S::dip(s)) {
+      f[s][0] = a[s].fx + T.alpha * (a[s].Ex * a[s].Exx + a[s].Ey * a[s].Exy + a[s].Ez * a[s].Exz);
+      f[s][1] = a[s].fy + T.alpha * (a[s].Ex * a[s].Exy + a[s].Ey * a[s].Eyy + a[s].Ez * a[s].Eyz);
+      f[s][2] = a[s].fz + T.alpha * (a[s].Ex * a[s].Exz + a[s].Ey * a[s].Eyz + a[s].Ez * a[s].Ezz);
+      U[s] = a[s].U - 0.5 * T.alpha * (a[s].Ex * a[s].Ex + a[s].Ey * a[s].Ey + a[s].Ez * a[s].Ez);
+    } else {
+      f[s][0] = a[s].fx;
+      f[s][1] = a[s].fy;
+      f[s][2] = a[s].fz;
+      U[s] = a[s].U;
+    }
+  }
+}
+
+// Verlet list of one group: every atom within R of the centre c0 it was built
+// around.  It serves while |c - c0| + off <= R - rlong, i.e. while every atom
+// that can be inside the longest cutoff of any site is on the list.
+struct VList {
+  double c0[3];
+  double R;
+  int n;
+  int valid;  // -1 none, 0 overflowed its capacity (walk the columns instead), 1 usable
+};
+
+template <bool LDS>
+using LIdx = typename std::conditional<LDS, uint16_t, int>::type;
+
+template <class S, int L, bool LDS>
+__device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, const double c[3],
+                           double R) {
+  RunGen g;
+  g.mode = 0;
+  g.c[0] = c[0];
+  g.c[1] = c[1];
+  g.c[2] = c[2];
+  g.R2 = R * R;
+  const int ix0 = clampi((int)floor((c[0] - R - T.cx) * T.inv_cell), 0, T.nx - 1);
+  const int ix1 = clampi((int)floor((c[0] + R - T.cx) * T.inv_cell), 0, T.nx - 1);
+  const int iy0 = clampi((int)floor((c[1] - R - T.cy) * T.inv_cell), 0, T.ny - 1);
+  const int iy1 = clampi((int)floor((c[1] + R - T.cy) * T.inv_cell), 0, T.ny - 1);
+  g.ix0 = ix0;
+  g.iy0 = iy0;
+  g.nys = iy1 - iy0 + 1;
+  const double R2 = R * R;
+  int n = 0;
+  // every lane of the group runs the same number of visits per chunk of L
+  // candidates (walk_runs strides t by L), so the ballot sees the whole group
+  for (int r0 = 0; r0 < (ix1 - ix0 + 1) * g.nys; r0 += L) {
+    int start = 0, len = 0, first = 1;
+    if (r0 + lane < (ix1 - ix0 + 1) * g.nys) g.get(T, r0 + lane, start, len, first);
+    const int incl = gscan_incl<L>(len, lane);
+    const int total = __shfl(incl, L - 1, L);
+    sh.incl[lane] = incl;
+    sh.base[lane] = start - (incl - len);
+    lds_fence();
+    int cur = 0;
+    for (int t0 = 0; t0 < total; t0 += L) {
+      const int t = t0 + lane;
+      bool keep = false;
+      int a = 0;
+      if (t < total) {
+        while (cur < L - 1 && t >= sh.incl[cur]) cur++;
+        a = sh.base[cur] + t;
+        const double4 at = LDS ? sh.atoms[a] : T.atoms[a];
+        const double ex = at.x - c[0], ey = at.y - c[1], ez = at.z - c[2];
+        keep = ex * ex + ey * ey + ez * ez < R2;
+      }
+      int below, cnt;
+      group_ballot<L>(keep, lane, below, cnt);
+      if (keep && n + below < T.list_cap) static_cast<LIdx<LDS> *>(sh.list)[n + below] = (LIdx<LDS>)a;
+      n += cnt;
+    }
+    lds_fence();
+  }
+  vl.c0[0] = c[0];
+  vl.c0[1] = c[1];
+  vl.c0[2] = c[2];
+  vl.R = R;
+  vl.n = n;
+  vl.valid = n <= T.list_cap;
+}
+
 // Forces and energies of all sites at positions p (group-uniform result).
 template <class S, int L, bool LDS, bool COUNT>
 __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p)[S::NS][3],
                          const double (&qs)[S::NS], double (&f)[S::NS][3], double (&U)[S::NS],
-                         Work &w) {
-  Acc res[S::NS];
+                         Work &w, VList *vl) {
+  constexpr bool CEN = any_central<S>();
 #pragma unroll
-  for (int s = 0; s < S::NS; s++) acc_zero(res[s]);
+  for (int s = 0; s < S::NS; s++) {
+    f[s][0] = f[s][1] = f[s][2] = 0.0;
+    U[s] = 0.0;
+  }
+  Acc a[S::NS];
+#pragma unroll
+  for (int s = 0; s < S::NS; s++) acc_zero(a[s]);
+  auto visit = [&](int ai, int first) {
+    double4 at;
+    double2 lp, lc;
+    fetch<LDS, CEN>(T, sh, ai, at, lp, lc);
+    pair_terms<S, COUNT>(T, p, qs, at, lp, lc, first, a, w);
+  };
   RunGen g;
   if (!S::CUT) {
     g.mode = 2;
-    Acc a[S::NS];
-#pragma unroll
-    for (int s = 0; s < S::NS; s++) acc_zero(a[s]);
-    sweep<S, L, LDS, COUNT>(T, sh, lane, g, 1, p, qs, a, w);
+    walk_runs<L>(T, sh, lane, g, 1, visit);
     reduce<S, L>(a);
-#pragma unroll
-    for (int s = 0; s < S::NS; s++) res[s] = a[s];
-  } else {
-    bool inbox = true;
-#pragma unroll
-    for (int s = 0; s < S::NS; s++) {
-      const int i = (int)floor((p[s][0] - T.cx) / T.cell);
-      const int j = (int)floor((p[s][1] - T.cy) / T.cell);
-      const int k = (int)floor((p[s][2] - T.cz) / T.cell);
-      inbox = inbox && i >= 0 && i < T.nx && j >= 0 && j < T.ny && k >= 0 && k < T.nz;
-    }
-    if (inbox) {
-      // centre of the probe: the site the others are symmetric about
-      const int cs = S::NS == 1 ? 0 : (S::NS == 3 && S::cen(1) ? 1 : S::NS - 1);
-      double off2 = 0.0;
-#pragma unroll
-      for (int s = 0; s < S::NS; s++) {
-        const double ex = p[s][0] - p[cs][0], ey = p[s][1] - p[cs][1], ez = p[s][2] - p[cs][2];
-        off2 = fmax(off2, ex * ex + ey * ey + ez * ez);
-      }
-      const double R = (any_long<S>() ? T.rc : T.rlj) + sqrt(off2) + 1e-6;
-      g.mode = 0;
-      g.c[0] = p[cs][0];
-      g.c[1] = p[cs][1];
-      g.c[2] = p[cs][2];
-      g.R2 = R * R;
-      const int ix0 = clampi((int)floor((g.c[0] - R - T.cx) / T.cell), 0, T.nx - 1);
-      const int ix1 = clampi((int)floor((g.c[0] + R - T.cx) / T.cell), 0, T.nx - 1);
-      const int iy0 = clampi((int)floor((g.c[1] - R - T.cy) / T.cell), 0, T.ny - 1);
-      const int iy1 = clampi((int)floor((g.c[1] + R - T.cy) / T.cell), 0, T.ny - 1);
-      g.ix0 = ix0;
-      g.iy0 = iy0;
-      g.nys = iy1 - iy0 + 1;
-      Acc a[S::NS];
-#pragma unroll
-      for (int s = 0; s < S::NS; s++) acc_zero(a[s]);
-      sweep<S, L, LDS, COUNT>(T, sh, lane, g, (ix1 - ix0 + 1) * g.nys, p, qs, a, w);
-      reduce<S, L>(a);
-#pragma unroll
-      for (int s = 0; s < S::NS; s++) res[s] = a[s];
-    } else {
-      // A site left the grid (exit step only): evaluate each site alone with
-      // the reference's stencils of its (possibly aliased) cell index.
-#pragma unroll
-      for (int ps = 0; ps < S::NS; ps++) {
-        if (!(S::lj(ps) || S::coul(ps) || S::dip(ps))) continue;
-        const int i = (int)floor((p[ps][0] - T.cx) / T.cell);
-        const int j = (int)floor((p[ps][1] - T.cy) / T.cell);
-        const int k = (int)floor((p[ps][2] - T.cz) / T.cell);
-        const long long idx = (long long)k + (long long)T.nz * j + (long long)T.nz * T.ny * i;
-        if (idx < 0 || idx >= T.ncells) continue;  // zero force, src/Force.cpp:58-66
-        g.mode = 1;
-        g.ci = (int)(idx / ((long long)T.ny * T.nz));
-        g.cj = (int)((idx / T.nz) % T.ny);
-        g.ck = (int)(idx % T.nz);
-        g.nfirst = T.n1;
-        const int nr = T.n1 + ((S::coul(ps) || S::dip(ps)) ? T.n2 : 0);
-        double pp[S::NS][3];
-#pragma unroll
-        for (int s = 0; s < S::NS; s++)
-#pragma unroll
-          for (int d = 0; d < 3; d++) pp[s][d] = (s == ps) ? p[s][d] : FAR;
-        Acc a[S::NS];
-#pragma unroll
-        for (int s = 0; s < S::NS; s++) acc_zero(a[s]);
-        sweep<S, L, LDS, COUNT>(T, sh, lane, g, nr, pp, qs, a, w);
-        reduce<S, L>(a);
-        res[ps] = a[ps];
-      }
-    }
+    finish<S>(T, a, -1, f, U);
+    return;
   }
+  // Exact zero: every site at least the longest cutoff away from the target's
+  // bounding box, so no atom passes any cutoff test (also for aliased cells).
+  bool near = false;
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
-    if (S::dip(s)) {
-      const Acc &a = res[s];
-      f[s][0] = a.fx + T.alpha * (a.Ex * a.Exx + a.Ey * a.Exy + a.Ez * a.Exz);
-      f[s][1] = a.fy + T.alpha * (a.Ex * a.Exy + a.Ey * a.Eyy + a.Ez * a.Eyz);
-      f[s][2] = a.fz + T.alpha * (a.Ex * a.Exz + a.Ey * a.Eyz + a.Ez * a.Ezz);
-      U[s] = a.U - 0.5 * T.alpha * (a.Ex * a.Ex + a.Ey * a.Ey + a.Ez * a.Ez);
-    } else {
-      f[s][0] = res[s].fx;
-      f[s][1] = res[s].fy;
-      f[s][2] = res[s].fz;
-      U[s] = res[s].U;
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double e = fmax(0.0, fmax(T.blo[d] - p[s][d], p[s][d] - T.bhi[d]));
+      d2 += e * e;
     }
+    near = near || d2 < T.rfar2;
+  }
+  if (!near) return;
+  bool inbox = true;
+#pragma unroll
+  for (int s = 0; s < S::NS; s++) {
+    const double i = floor((p[s][0] - T.cx) * T.inv_cell);
+    const double j = floor((p[s][1] - T.cy) * T.inv_cell);
+    const double k = floor((p[s][2] - T.cz) * T.inv_cell);
+    inbox = inbox && i >= 0.0 && i < T.nx && j >= 0.0 && j < T.ny && k >= 0.0 && k < T.nz;
+  }
+  if (inbox) {
+    constexpr int cs = S::centre();
+    double off2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < S::NS; s++) {
+      const double ex = p[s][0] - p[cs][0], ey = p[s][1] - p[cs][1], ez = p[s][2] - p[cs][2];
+      off2 = fmax(off2, ex * ex + ey * ey + ez * ez);
+    }
+    const double off = sqrt(off2);
+    const double c[3] = {p[cs][0], p[cs][1], p[cs][2]};
+    if (vl) {
+      const double e0 = c[0] - vl->c0[0], e1 = c[1] - vl->c0[1], e2 = c[2] - vl->c0[2];
+      const double moved = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+      if (vl->valid < 0 || moved + off + T.rlong + 1e-9 > vl->R) {
+        VList nl;
+        build_list<S, L, LDS>(T, sh, lane, nl, c, T.rlong + off + SKIN);
+        if (lane == 0) *vl = nl;
+        lds_fence();
+      }
+      const int n = vl->n;
+      if (vl->valid > 0) {
+        const LIdx<LDS> *lst = static_cast<const LIdx<LDS> *>(sh.list);
+        for (int t = lane; t < n; t += L) visit((int)lst[t], 1);
+        reduce<S, L>(a);
+        finish<S>(T, a, -1, f, U);
+        return;
+      }
+    }
+    // no (usable) list: walk the columns around the centre directly
+    const double R = T.rlong + off + 1e-6;
+    g.mode = 0;
+    g.c[0] = c[0];
+    g.c[1] = c[1];
+    g.c[2] = c[2];
+    g.R2 = R * R;
+    const int ix0 = clampi((int)floor((c[0] - R - T.cx) * T.inv_cell), 0, T.nx - 1);
+    const int ix1 = clampi((int)floor((c[0] + R - T.cx) * T.inv_cell), 0, T.nx - 1);
+    const int iy0 = clampi((int)floor((c[1] - R - T.cy) * T.inv_cell), 0, T.ny - 1);
+    const int iy1 = clampi((int)floor((c[1] + R - T.cy) * T.inv_cell), 0, T.ny - 1);
+    g.ix0 = ix0;
+    g.iy0 = iy0;
+    g.nys = iy1 - iy0 + 1;
+    walk_runs<L>(T, sh, lane, g, (ix1 - ix0 + 1) * g.nys, visit);
+    reduce<S, L>(a);
+    finish<S>(T, a, -1, f, U);
+    return;
+  }
+  // A site left the grid (exit step only): evaluate each site alone with the
+  // reference's stencils of its (possibly aliased) cell index.
+#pragma unroll
+  for (int ps = 0; ps < S::NS; ps++) {
+    if (!(S::lj(ps) || S::coul(ps) || S::dip(ps))) continue;
+    const int i = (int)floor((p[ps][0] - T.cx) / T.cell);
+    const int j = (int)floor((p[ps][1] - T.cy) / T.cell);
+    const int k = (int)floor((p[ps][2] - T.cz) / T.cell);
+    const long long idx = (long long)k + (long long)T.nz * j + (long long)T.nz * T.ny * i;
+    if (idx < 0 || idx >= T.ncells) continue;  // zero force, src/Force.cpp:58-66
+    g.mode = 1;
+    g.ci = (int)(idx / ((long long)T.ny * T.nz));
+    g.cj = (int)((idx / T.nz) % T.ny);
+    g.ck = (int)(idx % T.nz);
+    g.nfirst = T.n1;
+    const int nr = T.n1 + ((S::coul(ps) || S::dip(ps)) ? T.n2 : 0);
+    double pp[S::NS][3];
+#pragma unroll
+    for (int s = 0; s < S::NS; s++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) pp[s][d] = (s == ps) ? p[s][d] : FAR;
+    Acc b[S::NS];
+#pragma unroll
+    for (int s = 0; s < S::NS; s++) acc_zero(b[s]);
+    auto visit1 = [&](int ai, int first) {
+      double4 at;
+      double2 lp, lc;
+      fetch<LDS, CEN>(T, sh, ai, at, lp, lc);
+      pair_terms<S, COUNT>(T, pp, qs, at, lp, lc, first, b, w);
+    };
+    walk_runs<L>(T, sh, lane, g, nr, visit1);
+    reduce<S, L>(b);
+    finish<S>(T, b, ps, f, U);
   }
 }
 
@@ -598,15 +759,24 @@ __device__ bool setup(const Tgt &T, const double *u, double (&pos)[3][3], double
   return true;
 }
 
+// Per-try RATTLE coefficients (dt changes on every restart); each is the
+// value the reference computes inline: 0.5*dt/m, muij/(m*dt), muij/mi.
+struct Rat {
+  double hi, hj, ci, cj, pij;
+};
+__device__ __forceinline__ Rat rattle_coeffs(double mi, double mj, double dt) {
+  const double muij = mi * mj / (mi + mj);
+  return Rat{0.5 * dt / mi, 0.5 * dt / mj, muij / (mi * dt), muij / (mj * dt), muij / mi};
+}
+
 // RATTLE position stage, src/System.cpp:1700-1739
 __device__ __forceinline__ void rattle1(double *ri, double *rj, double *vi, double *vj,
-                                        const double *fi, const double *fj, double mi, double mj,
+                                        const double *fi, const double *fj, const Rat &c,
                                         double dt, double d2) {
-  const double muij = mi * mj / (mi + mj);
   double rci[3], rcj[3], vci[3], vcj[3], rcij[3], rij[3];
   for (int k = 0; k < 3; k++) {
-    vci[k] = vi[k] + 0.5 * dt / mi * fi[k] * KCONV;
-    vcj[k] = vj[k] + 0.5 * dt / mj * fj[k] * KCONV;
+    vci[k] = vi[k] + c.hi * fi[k] * KCONV;
+    vcj[k] = vj[k] + c.hj * fj[k] * KCONV;
     rci[k] = ri[k] + dt * vci[k];
     rcj[k] = rj[k] + dt * vcj[k];
     rij[k] = ri[k] - rj[k];
@@ -617,22 +787,22 @@ __device__ __forceinline__ void rattle1(double *ri, double *rj, double *vi, doub
   if (det < 0.0) det = 0.0;
   const double lam = (rr - sqrt(det)) / r2;
   for (int k = 0; k < 3; k++) {
-    vi[k] = vci[k] - muij / (mi * dt) * lam * rij[k];
-    vj[k] = vcj[k] + muij / (mj * dt) * lam * rij[k];
-    ri[k] = rci[k] - muij / mi * lam * rij[k];
-    rj[k] = rcj[k] + muij / mi * lam * rij[k];  // the reference uses mi here too
+    vi[k] = vci[k] - c.ci * lam * rij[k];
+    vj[k] = vcj[k] + c.cj * lam * rij[k];
+    ri[k] = rci[k] - c.pij * lam * rij[k];
+    rj[k] = rcj[k] + c.pij * lam * rij[k];  // the reference uses muij/mi for both sites
   }
 }
 
 // RATTLE velocity stage, src/System.cpp:1741-1767
 __device__ __forceinline__ void rattle2(const double *ri, const double *rj, double *vi, double *vj,
-                                        const double *fi, const double *fj, double mi, double mj,
-                                        double dt) {
+                                        const double *fi, const double *fj, const Rat &c,
+                                        double mi, double mj) {
   const double muij = mi * mj / (mi + mj);
   double rij[3], vcij[3], vci[3], vcj[3];
   for (int k = 0; k < 3; k++) {
-    vci[k] = vi[k] + 0.5 * dt / mi * fi[k] * KCONV;
-    vcj[k] = vj[k] + 0.5 * dt / mj * fj[k] * KCONV;
+    vci[k] = vi[k] + c.hi * fi[k] * KCONV;
+    vcj[k] = vj[k] + c.hj * fj[k] * KCONV;
     rij[k] = ri[k] - rj[k];
     vcij[k] = vci[k] - vcj[k];
   }
@@ -689,40 +859,64 @@ __device__ __forceinline__ void write_record(mccs_traj_record *rec, uint64_t j, 
   }
 }
 
-__device__ void stage(const Tgt &T, Shm &sh, char *smem, bool lds_atoms) {
-  // layout: atoms (double4) | lj rows | run scratch | types
+// Trajectory state of one group, kept in LDS between the phases of a step so
+// that only positions and force accumulators occupy registers during the
+// force evaluation.  Every lane of the group computes the same values; lane 0
+// stores them.
+struct GState {
+  double pos[3][3], vel[3][3], fs[3][3];
+  double rcm[3], vcm[3], rold[3], vcm_i[3];
+  double Ei, dt, dtheta, hdm, Utot;
+  Rat rc;
+  VList vl;
+  unsigned long long j;
+  int status, tries, max_tries, steps;
+};
+
+// Block LDS layout: atoms and LJ pairs (when staged) | per-lane run scratch
+// (2 ints per thread) | one GState per group | one Verlet list per group.
+__host__ __device__ inline size_t lds_bytes(int natoms, int list_cap, int block, int lanes,
+                                            bool lds_atoms, bool central) {
+  const size_t lidx = lds_atoms ? sizeof(uint16_t) : sizeof(int);
+  size_t b = 0;
+  if (lds_atoms)
+    b += (sizeof(double4) + sizeof(double2) * (central ? 2 : 1)) * (size_t)natoms;
+  b += sizeof(int) * 2 * (size_t)block;
+  b += sizeof(GState) * (size_t)(block / lanes);
+  b += lidx * (size_t)list_cap * (size_t)(block / lanes);
+  return b + 64;
+}
+
+template <int L, bool LDS, bool CEN>
+__device__ void stage(const Tgt &T, Shm &sh, char *smem) {
   char *p = smem;
   double4 *atoms = reinterpret_cast<double4 *>(p);
-  if (lds_atoms) p += sizeof(double4) * (size_t)T.natoms;
-  double *lj = reinterpret_cast<double *>(p);
-  p += sizeof(double) * 10 * (size_t)T.ntypes;
+  if (LDS) p += sizeof(double4) * (size_t)T.natoms;
+  double2 *ljp = reinterpret_cast<double2 *>(p);
+  if (LDS) p += sizeof(double2) * (size_t)T.natoms;
+  double2 *ljc = reinterpret_cast<double2 *>(p);
+  if (LDS && CEN) p += sizeof(double2) * (size_t)T.natoms;
   int *scratch = reinterpret_cast<int *>(p);
   p += sizeof(int) * 2 * blockDim.x;
-  uint8_t *type = reinterpret_cast<uint8_t *>(p);
-  for (int i = threadIdx.x; i < 10 * T.ntypes; i += blockDim.x) lj[i] = T.lj[i];
-  if (lds_atoms) {
+  GState *states = reinterpret_cast<GState *>(p);
+  p += sizeof(GState) * (blockDim.x / L);
+  char *lists = p;
+  if (LDS) {
     for (int i = threadIdx.x; i < T.natoms; i += blockDim.x) {
       atoms[i] = T.atoms[i];
-      type[i] = T.type[i];
+      ljp[i] = T.ljp[i];
+      if (CEN) ljc[i] = T.ljc[i];
     }
   }
   __syncthreads();
-  sh.lj = lj;
-  sh.atoms = atoms;
-  sh.type = type;
-  sh.incl = scratch;
-  sh.base = scratch;
-}
-
-template <int L>
-__device__ __forceinline__ void group_scratch(Shm &sh, char *smem, const Tgt &T, bool lds_atoms) {
-  char *p = smem;
-  if (lds_atoms) p += sizeof(double4) * (size_t)T.natoms;
-  p += sizeof(double) * 10 * (size_t)T.ntypes;
-  int *scratch = reinterpret_cast<int *>(p);
   const int g = threadIdx.x / L;
+  sh.atoms = atoms;
+  sh.ljp = ljp;
+  sh.ljc = ljc;
   sh.incl = scratch + 2 * L * g;
   sh.base = scratch + 2 * L * g + L;
+  sh.state = states + g;
+  sh.list = lists + sizeof(LIdx<LDS>) * (size_t)T.list_cap * g;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -731,177 +925,257 @@ __device__ __forceinline__ void group_scratch(Shm &sh, char *smem, const Tgt &T,
 // group performs exactly one force evaluation: the initial force of a new
 // (or restarted) trajectory, or the force of one velocity-Verlet/RATTLE step.
 template <int GK, int FT, int L, bool LDS, bool COUNT>
-__global__ __launch_bounds__(512) void traj_kernel(Tgt T, const double *__restrict__ u, uint64_t n,
-                                                   mccs_traj_record *__restrict__ rec,
-                                                   unsigned long long *__restrict__ queue) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void traj_kernel(
+    Tgt T, const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec,
+    unsigned long long *__restrict__ queue) {
   using S = Spec<GK, FT>;
+  constexpr bool CEN = any_central<S>();
   extern __shared__ __align__(16) char smem[];
   Shm sh;
-  stage(T, sh, smem, LDS);
-  group_scratch<L>(sh, smem, T, LDS);
+  stage<L, LDS, CEN>(T, sh, smem);
   const int lane = threadIdx.x % L;
+  GState &st = *sh.state;
+  constexpr int PA = 0, PB = GK == 2 ? 1 : 2;  // the constrained pair (N-N or O-O)
+  constexpr double MA = S::NS == 1 ? 1.0 : S::m(PA), MB = S::NS == 1 ? 1.0 : S::m(PB);
+  const double iea = 1.0 / T.ea, ieb = 1.0 / T.eb, iec = 1.0 / T.ec;
 
   double qs[S::NS];
 #pragma unroll
   for (int s = 0; s < S::NS; s++) qs[s] = S::q(s);
 
-  int status = ST_NEED;
-  uint64_t j = 0;
-  int tries = 0, max_tries = 8, steps = 0;
-  double dt = T.dt, dtheta = 0.0, Ei = 0.0;
-  double pos[3][3], vel[3][3], fs[3][3], Us[3];
-  double rcm[3], vcm[3], rold[3], vcm_i[3];
   Work w{0, 0, 0, 0};
+  int status = ST_NEED;
+  if (lane == 0) {
+    st.status = ST_NEED;
+    st.vl.valid = -1;
+  }
+  lds_fence();
 
   while (status != ST_DONE) {
     // ---- phase A: new trajectory, restart, or first half-step
-    while (status == ST_NEED) {
-      unsigned long long t = 0;
-      if (lane == 0) t = atomicAdd(queue, 1ull);
-      t = __shfl(t, 0, L);
-      if (t >= n) {
+    if (status == ST_NEED || status == ST_RESTART) {
+      double pos[3][3], vel[3][3], rcm[3], vcm[3];
+      uint64_t j = st.j;
+      bool ok = false;
+      if (status == ST_RESTART) {
+        setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);
+        ok = true;
+      }
+      while (!ok) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(queue, 1ull);
+        t = __shfl(t, 0, L);
+        if (t >= n) break;
+        j = t;
+        w = Work{0, 0, 0, 0};
+        if (setup<S>(T, u + 9 * j, pos, vel, rcm, vcm)) ok = true;
+        else write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, T.bmax, w, COUNT);
+      }
+      if (!ok) {
         status = ST_DONE;
         break;
       }
-      j = t;
-      w = Work{0, 0, 0, 0};
-      if (setup<S>(T, u + 9 * j, pos, vel, rcm, vcm)) {
-        status = ST_INIT;
-      } else {
-        write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, T.bmax, w, COUNT);
+      if (lane == 0) {
+        for (int s = 0; s < S::NS; s++)
+          for (int d = 0; d < 3; d++) {
+            st.pos[s][d] = pos[s][d];
+            st.vel[s][d] = vel[s][d];
+          }
+        for (int d = 0; d < 3; d++) {
+          st.rcm[d] = rcm[d];
+          st.vcm[d] = vcm[d];
+        }
+        st.j = j;
       }
-    }
-    if (status == ST_DONE) break;
-    if (status == ST_RESTART) {
-      setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);
-    } else if (status == ST_STEP) {
+      status = status == ST_RESTART ? ST_RESTART : ST_INIT;
+    } else {  // ST_STEP: first half
+      const double dt = st.dt;
+      double rcm[3];
       if (S::NS == 1) {
+        const double hdm = st.hdm;
+        double v[3], x[3];
         for (int d = 0; d < 3; d++) {
-          vel[0][d] += 0.5 * dt / T.mu * fs[0][d] * KCONV;
-          pos[0][d] += dt * vel[0][d];
-          rcm[d] = pos[0][d];
+          v[d] = st.vel[0][d] + hdm * st.fs[0][d] * KCONV;
+          x[d] = st.pos[0][d] + dt * v[d];
+          rcm[d] = x[d];
         }
+        if (lane == 0)
+          for (int d = 0; d < 3; d++) {
+            st.vel[0][d] = v[d];
+            st.pos[0][d] = x[d];
+          }
       } else {
-        const int a = 0, b = GK == 2 ? 1 : 2;  // the constrained pair (N-N or O-O)
-        double fi[3], fj[3];
-        pair_forces<S, GK>(fs, fi, fj);
-        rattle1(pos[a], pos[b], vel[a], vel[b], fi, fj, S::m(a), S::m(b), dt, T.d2);
-        const int mid = GK == 2 ? 2 : 1;
+        double fs[3][3], ri[3], rj[3], vi[3], vj[3], fi[3], fj[3];
+        for (int s = 0; s < 3; s++)
+          for (int d = 0; d < 3; d++) fs[s][d] = st.fs[s][d];
         for (int d = 0; d < 3; d++) {
-          rcm[d] = 0.5 * (pos[a][d] + pos[b][d]);
-          vcm[d] = 0.5 * (vel[a][d] + vel[b][d]);
-          pos[mid][d] = rcm[d];
+          ri[d] = st.pos[PA][d]; rj[d] = st.pos[PB][d];
+          vi[d] = st.vel[PA][d]; vj[d] = st.vel[PB][d];
+        }
+        pair_forces<S, GK>(fs, fi, fj);
+        const Rat rc = st.rc;
+        rattle1(ri, rj, vi, vj, fi, fj, rc, dt, T.d2);
+        constexpr int mid = GK == 2 ? 2 : 1;
+        double vcm[3];
+        for (int d = 0; d < 3; d++) {
+          rcm[d] = 0.5 * (ri[d] + rj[d]);
+          vcm[d] = 0.5 * (vi[d] + vj[d]);
+        }
+        if (lane == 0)
+          for (int d = 0; d < 3; d++) {
+            st.pos[PA][d] = ri[d]; st.pos[PB][d] = rj[d];
+            st.vel[PA][d] = vi[d]; st.vel[PB][d] = vj[d];
+            st.pos[mid][d] = rcm[d];
+            st.vcm[d] = vcm[d];
+          }
+      }
+      double rold[3];
+      for (int d = 0; d < 3; d++) rold[d] = st.rold[d];
+      const double dth = st.dtheta + anglevec(rcm, rold);
+      if (lane == 0) {
+        st.dtheta = dth;
+        for (int d = 0; d < 3; d++) {
+          st.rold[d] = rcm[d];
+          st.rcm[d] = rcm[d];
         }
       }
-      dtheta += anglevec(rcm, rold);
-      for (int d = 0; d < 3; d++) rold[d] = rcm[d];
     }
+    lds_fence();
 
     // ---- phase B: one force evaluation for every live group
     {
       double p[S::NS][3], f[S::NS][3], U[S::NS];
 #pragma unroll
       for (int s = 0; s < S::NS; s++)
-        for (int d = 0; d < 3; d++) p[s][d] = pos[s][d];
-      evaluate<S, L, LDS, COUNT>(T, sh, lane, p, qs, f, U, w);
+        for (int d = 0; d < 3; d++) p[s][d] = st.pos[s][d];
+      evaluate<S, L, LDS, COUNT>(T, sh, lane, p, qs, f, U, w, &st.vl);
+      double Ut = 0.0;
 #pragma unroll
-      for (int s = 0; s < S::NS; s++) {
-        Us[s] = U[s];
-        for (int d = 0; d < 3; d++) fs[s][d] = f[s][d];
+      for (int s = 0; s < S::NS; s++) Ut += U[s];
+      if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < S::NS; s++)
+          for (int d = 0; d < 3; d++) st.fs[s][d] = f[s][d];
+        st.Utot = Ut;
       }
     }
-    double Utot = 0.0;
-#pragma unroll
-    for (int s = 0; s < S::NS; s++) Utot += Us[s];
+    lds_fence();
 
     // ---- phase C: bookkeeping, second half-step, exit tests
     if (status == ST_INIT || status == ST_RESTART) {
-      if (status == ST_INIT) {
-        double Ek = 0.0;
-        if (S::NS == 1) Ek = kinetic(T.mu, vel[0]);
-        else
-          for (int s = 0; s < S::NS; s++) Ek += kinetic(S::m(s), vel[s]);
-        Ei = Utot + Ek;
-        tries = 0;
-        max_tries = 8;
-        steps = 0;
-        dt = T.dt;
-        for (int d = 0; d < 3; d++) vcm_i[d] = vcm[d];
-        if (GK == 3) vcm_i[0] = vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
-      }
-      dtheta = 0.0;
-      for (int d = 0; d < 3; d++) rold[d] = rcm[d];
-      if (GK == 3) {  // CO2 restarts from vcm_i (src/System.cpp:1134)
-        for (int d = 0; d < 3; d++) vcm[d] = vcm_i[d];
+      if (lane == 0) {
+        if (status == ST_INIT) {
+          double Ek = 0.0;
+          if (S::NS == 1) Ek = kinetic(T.mu, st.vel[0]);
+          else  // the N2 dummy is massless and adds nothing
+            for (int s = 0; s < (GK == 2 ? 2 : 3); s++) Ek += kinetic(S::m(s), st.vel[s]);
+          st.Ei = st.Utot + Ek;
+          st.tries = 0;
+          st.max_tries = 8;
+          st.steps = 0;
+          st.dt = T.dt;
+          st.hdm = 0.5 * T.dt / T.mu;
+          st.rc = rattle_coeffs(MA, MB, T.dt);
+          for (int d = 0; d < 3; d++) st.vcm_i[d] = st.vcm[d];
+          if (GK == 3) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
+        }
+        st.dtheta = 0.0;
+        for (int d = 0; d < 3; d++) st.rold[d] = st.rcm[d];
       }
       status = ST_STEP;
+      lds_fence();
       continue;
     }
     // ST_STEP: second half
+    double vcm[3], Ek = 0.0;
     if (S::NS == 1) {
-      for (int d = 0; d < 3; d++) {
-        vel[0][d] += 0.5 * dt / T.mu * fs[0][d] * KCONV;
-        vcm[d] = vel[0][d];
-      }
+      const double hdm = st.hdm;
+      for (int d = 0; d < 3; d++) vcm[d] = st.vel[0][d] + hdm * st.fs[0][d] * KCONV;
+      if (lane == 0)
+        for (int d = 0; d < 3; d++) st.vel[0][d] = vcm[d];
+      Ek = kinetic(T.mu, vcm);
     } else {
-      const int a = 0, b = GK == 2 ? 1 : 2;
-      double fi[3], fj[3];
+      double fs[3][3], ri[3], rj[3], vi[3], vj[3], fi[3], fj[3];
+      for (int s = 0; s < 3; s++)
+        for (int d = 0; d < 3; d++) fs[s][d] = st.fs[s][d];
+      for (int d = 0; d < 3; d++) {
+        ri[d] = st.pos[PA][d]; rj[d] = st.pos[PB][d];
+        vi[d] = st.vel[PA][d]; vj[d] = st.vel[PB][d];
+      }
       pair_forces<S, GK>(fs, fi, fj);
-      rattle2(pos[a], pos[b], vel[a], vel[b], fi, fj, S::m(a), S::m(b), dt);
-      for (int d = 0; d < 3; d++) vcm[d] = 0.5 * (vel[a][d] + vel[b][d]);
-      if (GK == 3)
-        for (int d = 0; d < 3; d++) vel[1][d] = vcm[d];
+      const Rat rc = st.rc;
+      rattle2(ri, rj, vi, vj, fi, fj, rc, MA, MB);
+      for (int d = 0; d < 3; d++) vcm[d] = 0.5 * (vi[d] + vj[d]);
+      if (lane == 0)
+        for (int d = 0; d < 3; d++) {
+          st.vel[PA][d] = vi[d];
+          st.vel[PB][d] = vj[d];
+          st.vcm[d] = vcm[d];
+          if (GK == 3) st.vel[1][d] = vcm[d];
+        }
+      // exit-test kinetic energy: N2 over its two atoms, CO2 over all three
+      // sites with the carbon moving at vcm (src/System.cpp:880-886, :1323-1329)
+      Ek = kinetic(MA, vi) + kinetic(MB, vj);
+      if (GK == 3) Ek = kinetic(S::m(0), vi) + kinetic(S::m(1), vcm) + kinetic(S::m(2), vj);
     }
-    steps++;
-    bool restart = false;
+    double rcm[3];
+    for (int d = 0; d < 3; d++) rcm[d] = st.rcm[d];
+    const int steps = st.steps + 1;
+    int tries = st.tries, max_tries = st.max_tries;
+    bool restart = false, finished = false;
+    int success = 0;
+    double chi = 0.0;
     if (steps >= MAX_STEPS || !isfinite(rcm[0] + rcm[1] + rcm[2] + vcm[0] + vcm[1] + vcm[2])) {
       // the reference would loop forever here (zero speed or non-finite
       // state); the trajectory is reported lost with tries = -1
-      write_record<L>(rec, j, lane, 1, 0, -1, steps, 0.0, T.bmax, w, COUNT);
-      status = ST_NEED;
-      continue;
-    }
-    if (dtheta > 1.5 * PI) {
+      finished = true;
+      tries = -1;
+    } else if (st.dtheta > 1.5 * PI) {
       tries++;
-      dt = T.dt / (tries + 1);
       restart = true;
     } else if (fabs(rcm[0]) >= T.hx || fabs(rcm[1]) >= T.hy || fabs(rcm[2]) >= T.hz ||
-               (rcm[0] / T.ea) * (rcm[0] / T.ea) + (rcm[1] / T.eb) * (rcm[1] / T.eb) +
-                       (rcm[2] / T.ec) * (rcm[2] / T.ec) >
+               (rcm[0] * iea) * (rcm[0] * iea) + (rcm[1] * ieb) * (rcm[1] * ieb) +
+                       (rcm[2] * iec) * (rcm[2] * iec) >
                    1.0) {
-      double Ek = 0.0, dE;
-      if (S::NS == 1) {
-        Ek = kinetic(T.mu, vcm);
-        dE = fabs(Utot + Ek - Ei) / Ei * 100.0;
-      } else {
-        const int ns_ke = GK == 2 ? 2 : 3;  // N2 excludes the massless dummy
-        for (int s = 0; s < ns_ke; s++) Ek += kinetic(S::m(s), vel[s]);
-        dE = fabs((Utot + Ek - Ei) / Ei) * 100.0;
-      }
+      const double Ei = st.Ei, Ut = st.Utot;
+      const double dE = S::NS == 1 ? fabs(Ut + Ek - Ei) / Ei * 100.0
+                                   : fabs((Ut + Ek - Ei) / Ei) * 100.0;
       const double tol = GK == 1 ? 0.5 : (GK == 2 ? 1.0 : 20.0);
       if (dE > tol) {
         tries++;
         max_tries++;
-        dt = T.dt / (tries + 1);
         restart = true;
       } else {
-        const double chi = anglevec(vcm_i, vcm);
-        write_record<L>(rec, j, lane, 1, 1, tries, steps, chi, T.bmax, w, COUNT);
-        status = ST_NEED;
-        continue;
+        double vi[3];
+        for (int d = 0; d < 3; d++) vi[d] = st.vcm_i[d];
+        chi = anglevec(vi, vcm);
+        success = 1;
+        finished = true;
       }
     }
-    if (restart) {
-      if (tries < max_tries && max_tries < 10) {
-        status = ST_RESTART;
-      } else {
-        write_record<L>(rec, j, lane, 1, 0, tries, steps, 0.0, T.bmax, w, COUNT);
-        status = ST_NEED;
+    if (restart && !(tries < max_tries && max_tries < 10)) finished = true;  // lost
+    const uint64_t j = st.j;
+    lds_fence();
+    if (finished) {
+      write_record<L>(rec, j, lane, 1, success, tries, steps, chi, T.bmax, w, COUNT);
+      status = ST_NEED;
+    } else if (restart) {
+      const double dt = T.dt / (tries + 1);
+      if (lane == 0) {
+        st.tries = tries;
+        st.max_tries = max_tries;
+        st.steps = steps;
+        st.dt = dt;
+        st.hdm = 0.5 * dt / T.mu;
+        st.rc = rattle_coeffs(MA, MB, dt);
       }
+      status = ST_RESTART;
+    } else {
+      if (lane == 0) st.steps = steps;
     }
+    lds_fence();
   }
 }
-
 
 using TrajFn = void (*)(Tgt, const double *, uint64_t, mccs_traj_record *, unsigned long long *);
 

@@ -513,6 +513,12 @@ static double mt_uniform(mt_t *m) {
   return ((double)mt_next(m) - 0.0) / 4294967295.0;
 }
 
+void oracle_mt_raw(uint32_t seed, int64_t n, uint32_t *out) {
+  mt_t m;
+  mt_seed(&m, seed);
+  for (int64_t i = 0; i < n; i++) out[i] = mt_next(&m);
+}
+
 /* Sample pre-generation, src/System.cpp:184-197 */
 void oracle_samples(uint32_t seed, int multi, double bmax, int64_t n, double *u) {
   mt_t m;
@@ -726,6 +732,14 @@ static inline double ellipsoid_r(const oracle_params *p, const double r[3]) {
   return dot3(ur, ur);
 }
 
+/* The reference loops forever for a probe that never leaves (zero speed or a
+ * non-finite state); here such a trajectory is reported lost with tries = -1
+ * after this many steps, as the engine does. */
+#define ORACLE_MAX_STEPS 4000000
+#define CAPPED(o, r, v)                                                        \
+  ((o)->steps >= ORACLE_MAX_STEPS ||                                            \
+   !isfinite((r)[0] + (r)[1] + (r)[2] + (v)[0] + (v)[1] + (v)[2]))
+
 /* System::run_He, src/System.cpp:338-511 (also Ar and 1-site co2) */
 static void run_atomic(const oracle_ctx *c, const double pos0[3],
                        const double vel0[3], oracle_record *o) {
@@ -764,6 +778,11 @@ static void run_atomic(const oracle_ctx *c, const double pos0[3],
     for (int i = 0; i < 3; i++) vcm[i] += 0.5 * dt / mu * f[i] * KCALMOLANGAMU_TO_ANGFS2;
     stepcount++;
     o->steps++;
+    if (CAPPED(o, rcm, vcm)) {
+      o->success = 0;
+      o->tries = -1;
+      return;
+    }
     if (dtheta > theta_max) {
       trajTries++;
       dt = time_step / (trajTries + 1);
@@ -795,7 +814,7 @@ static void run_atomic(const oracle_ctx *c, const double pos0[3],
 static void run_n2(const oracle_ctx *c, const double pos0[9], const double vel0[9],
                    const double rcm0[3], const double vcm0[3], oracle_record *o) {
   const oracle_params *p = &c->p;
-  const double time_step = p->dt, mu = p->mu;
+  const double time_step = p->dt;
   double dt = time_step;
   int maxTries = 8, trajTries = 0, stepcount = 0;
   const double theta_max = 1.5 * M_PI;
@@ -870,6 +889,11 @@ static void run_n2(const oracle_ctx *c, const double pos0[9], const double vel0[
     }
     stepcount++;
     o->steps++;
+    if (CAPPED(o, rcm, vcm)) {
+      o->success = 0;
+      o->tries = -1;
+      return;
+    }
     if (dtheta > theta_max) {
       trajTries++;
       dt = time_step / (trajTries + 1);
@@ -916,6 +940,7 @@ static void run_co2(const oracle_ctx *c, const double pos0[9], const double vel0
   memcpy(vxi, v, sizeof v);
   const double *m = c->site_m;
   const double M = c->gas_mass;
+  (void)p;
   double rcm[3], vcm[3], rcm_i[3], vcm_i[3], rcm_old[3];
   memcpy(rcm, rcm0, sizeof rcm);
   vcm[0] = vcm0[1];
@@ -985,6 +1010,11 @@ static void run_co2(const oracle_ctx *c, const double pos0[9], const double vel0
     }
     stepcount++;
     o->steps++;
+    if (CAPPED(o, rcm, vcm)) {
+      o->success = 0;
+      o->tries = -1;
+      return;
+    }
     if (dtheta > theta_max) {
       trajTries++;
       dt = time_step / (trajTries + 1);

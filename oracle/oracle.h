@@ -97,6 +97,9 @@ void oracle_site_force(const oracle_ctx *ctx, int kernel, const double r[3],
 void oracle_samples(uint32_t seed, int multi_site, double bmax, int64_t n,
                     double *u);
 
+/* Raw mt19937 outputs (for the standard's known-answer check). */
+void oracle_mt_raw(uint32_t seed, int64_t n, uint32_t *out);
+
 /* Maxwell-Boltzmann speed by the reference's sequential scan
  * (System::velGenerator, src/System.cpp:1537-1555). */
 double oracle_vel_generator(double m, double temperature, double u);

@@ -77,6 +77,7 @@ def lib():
                                         C.POINTER(C.c_double)]
         L.oracle_samples.argtypes = [C.c_uint32, C.c_int, C.c_double, C.c_int64,
                                      C.POINTER(C.c_double)]
+        L.oracle_mt_raw.argtypes = [C.c_uint32, C.c_int64, C.POINTER(C.c_uint32)]
         L.oracle_vel_generator.restype = C.c_double
         L.oracle_vel_generator.argtypes = [C.c_double, C.c_double, C.c_double]
         L.oracle_run.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
