@@ -165,6 +165,7 @@ def main():
     wall = time.perf_counter() - t0
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
 
+    executed = E.last_work()  # every try run, including side-by-side tries left unused
     r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
     me = mean_err.cpu().numpy()
     hits = r["hit"] == 1
@@ -187,6 +188,8 @@ def main():
                    "parallelism": "1 GPU"},
         "force_evals_per_s": round(site_evals / (kern_ms * 1e-3), 1),
         "ccs": {"mean": float(me[0]), "err": float(me[1])},
+        "steps_per_launch": {"of_records": int(r["steps"].sum()),
+                             "executed": int(executed["steps"])},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": None, "kernel": "traj_kernel<N2, type 6>",

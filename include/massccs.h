@@ -130,6 +130,10 @@ typedef struct {
   int32_t block;           /* threads per block: 0 = auto, else 128 or 256 */
   int32_t use_lds;         /* -1 auto, 0 global-memory atoms, 1 atoms staged in LDS */
   int32_t count_work;      /* 1: fill the work counters of mccs_traj_record */
+  int32_t spec_below;      /* tries of a restarting trajectory run one after another; once
+                              fewer than this many trajectories are unresolved and no new
+                              one is left, idle groups also start its later tries side by
+                              side (results are the same either way): -1 = auto, 0 = never */
 } mccs_engine_opts;
 
 mccs_status mccs_engine_create(const mccs_prepared *p, const mccs_engine_opts *opts,
@@ -159,6 +163,14 @@ mccs_status mccs_run(mccs_engine *e, const double *samples, uint64_t n,
  * a HIP stream (NULL = the engine's stream) without synchronising. */
 mccs_status mccs_run_device(mccs_engine *e, const double *samples_dev, uint64_t n,
                             mccs_traj_record *records_dev, void *stream);
+
+/* Work the device executed in the last mccs_run / mccs_run_device on this
+ * engine, every try included (a trajectory that restarts has its later tries
+ * run concurrently; tries made moot by an earlier success are abandoned):
+ * totals[0..4] = (site, atom) pairs examined, LJ pairs, Coulomb pairs,
+ * induced-dipole pairs, integration steps.  Synchronises the stream of that
+ * run.  Calls on one engine must not overlap (they share its work queue). */
+mccs_status mccs_engine_last_work(mccs_engine *e, uint64_t totals[5]);
 
 /* Site roles for mccs_eval_sites.  For N2: MCCS_SITE_N uses
  * Force::lennardjones_coulomb[_LC] (or lennardjones[_LC] for types 1/2) with

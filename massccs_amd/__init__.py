@@ -56,7 +56,7 @@ class PreparedInfo(C.Structure):
 
 class EngineOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("block", C.c_int32),
-                ("use_lds", C.c_int32), ("count_work", C.c_int32)]
+                ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32)]
 
 
 RECORD_DTYPE = np.dtype([("hit", "<i4"), ("success", "<i4"), ("tries", "<i4"), ("steps", "<i4"),
@@ -69,7 +69,7 @@ EXPORTS = [
     "mccs_prepare_atoms", "mccs_prepared_destroy", "mccs_prepared_get_info",
     "mccs_prepared_get_atoms", "mccs_prepared_report", "mccs_samples", "mccs_engine_create",
     "mccs_engine_destroy", "mccs_run", "mccs_run_device", "mccs_eval_sites", "mccs_tally",
-    "mccs_tally_device", "mccs_version",
+    "mccs_tally_device", "mccs_version", "mccs_engine_last_work",
 ]
 
 _lib = None
@@ -107,6 +107,7 @@ def lib():
     L.mccs_run_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
     L.mccs_eval_sites.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mccs_engine_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.mccs_tally.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                              C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_tally_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
@@ -204,8 +205,9 @@ class Prepared:
 class Engine:
     """Device engine on one GPU (target resident in HBM)."""
 
-    def __init__(self, prepared, device=0, lanes=0, block=0, use_lds=-1, count_work=True):
-        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0)
+    def __init__(self, prepared, device=0, lanes=0, block=0, use_lds=-1, count_work=True,
+                 spec_below=-1):
+        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below)
         h = C.c_void_p()
         _check(lib().mccs_engine_create(prepared.h, C.byref(o), C.byref(h)))
         self.h, self.prepared = h, prepared
@@ -233,6 +235,12 @@ class Engine:
                                        C.c_void_p(omega_ptr), C.c_void_p(counts_ptr),
                                        C.c_void_p(mean_err_ptr),
                                        C.c_void_p(stream) if stream else None))
+
+    def last_work(self):
+        """Executed totals of the last run: visits, lj, coul, dip, steps."""
+        out = (C.c_uint64 * 5)()
+        _check(lib().mccs_engine_last_work(self.h, out))
+        return dict(zip(("visits", "lj", "coul", "dip", "steps"), list(out)))
 
     def eval_sites(self, site, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

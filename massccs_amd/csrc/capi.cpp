@@ -157,6 +157,7 @@ mccs_status mccs_engine_create(const mccs_prepared *pp, const mccs_engine_opts *
     *out = nullptr;
     mccs_engine_opts o{};
     o.use_lds = -1;
+    o.spec_below = -1;
     if (opts) o = *opts;
     auto *e = mccs::engine_create(*pp->p, o);
     *out = new mccs_engine{e};
@@ -189,6 +190,13 @@ mccs_status mccs_eval_sites(mccs_engine *e, int32_t site, const double *xyz, uin
   return guard([&] {
     need(e && ((xyz && f && U) || n == 0), "null argument");
     mccs::engine_eval_sites(e->e, site, xyz, n, f, U);
+  });
+}
+
+mccs_status mccs_engine_last_work(mccs_engine *e, uint64_t totals[5]) {
+  return guard([&] {
+    need(e && totals, "null argument");
+    mccs::engine_last_work(e->e, totals);
   });
 }
 

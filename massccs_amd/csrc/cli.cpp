@@ -40,6 +40,7 @@ int main(int argc, char **argv) {
 
   mccs_engine_opts opts{};
   opts.use_lds = -1;
+  opts.spec_below = -1;
   if (const char *d = std::getenv("MASSCCS_DEVICE")) opts.device = std::atoi(d);
   mccs_engine *eng = nullptr;
   st = mccs_engine_create(prep, &opts, &eng);

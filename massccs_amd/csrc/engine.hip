@@ -67,9 +67,6 @@ __global__ void tally_final_kernel(const double *__restrict__ omega, uint64_t n_
   mean_err[1] = sqrt(sig2 / (double)(float)n_iter);
 }
 
-__global__ void zero_u64(unsigned long long *p) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
-}
 
 // ---------------------------------------------------------------- dispatch
 using EvalFn = void (*)(Tgt, const double *, uint64_t, double, double *, double *);
@@ -113,6 +110,10 @@ EvalFn select_eval(bool lj, bool cen, bool coul, bool dip, bool cut, bool lds) {
 using namespace dev;
 
 // ---------------------------------------------------------------- engine
+constexpr int QLINE = 32;  // 64-bit words per counter line
+constexpr int QCTL_WORDS = 6 * QLINE;
+constexpr uint64_t SLOT_SLACK = 8192;  // spawn slots beyond 8 per trajectory
+
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -122,7 +123,15 @@ struct Engine {
   Tgt T{};
   double site_q[5] = {0, -0.4825, 0.965, -0.3256, 0.6512};
   std::vector<void *> allocs;
-  unsigned long long *queue = nullptr;
+  // work queue: counters, each polled one on its own 256-byte line (ticket,
+  // shead, stail, active, work[5], stalled), and a per-trajectory workspace
+  // grown on demand
+  unsigned long long *qctl = nullptr;
+  char *qws = nullptr;
+  uint64_t qcap = 0;
+  unsigned long long wait_ticks = 0;
+  int spec_below = -1;
+  hipStream_t last_stream = nullptr;
   int nsites = 1;
   void *dalloc(size_t bytes) {
     void *p = nullptr;
@@ -162,6 +171,7 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   E->nsites = P.gas.nsites;
   if (E->gk == 1 && E->ft > 4) E->ft = 2;
   E->count = o.count_work != 0;
+  E->spec_below = o.spec_below;
 
   // per-atom LJ pairs in the reference's order (src/Force.cpp:84-85):
   // lj1 = 4 eps sig^6, lj2 = lj1 sig^6; the second pair for the CO2 carbon site
@@ -229,7 +239,14 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
     T.rfar2 = rf * rf * (1.0 + 1e-12);
     T.rlong = longr ? rc : rlj;
   }
-  E->queue = reinterpret_cast<unsigned long long *>(E->dalloc(sizeof(unsigned long long)));
+  E->qctl = reinterpret_cast<unsigned long long *>(E->dalloc(sizeof(unsigned long long) * QCTL_WORDS));
+  HIP_OK(hipMemset(E->qctl, 0, sizeof(unsigned long long) * QCTL_WORDS));
+  {  // a group waits for work at most 120 s (any single try takes far less)
+    int khz = 0;
+    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E->device));
+    if (khz <= 0) khz = 100000;
+    E->wait_ticks = 120ull * 1000ull * (unsigned long long)khz;
+  }
 
   // launch shape: lanes per trajectory by target size unless given
   int L = o.lanes;
@@ -283,11 +300,60 @@ void engine_run_device(Engine *E, const double *u_dev, uint64_t n, mccs_traj_rec
   const uint64_t groups_per_block = E->block / E->lanes;
   uint64_t blocks = (uint64_t)per_cu * E->cus;
   blocks = std::min<uint64_t>(blocks, (n + groups_per_block - 1) / groups_per_block);
-  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, s, E->queue);
+  // per trajectory: TRIES_MAX try outcomes, 8 spawn slots, claimed tries,
+  // spawned and resolved flags
+  constexpr size_t per_traj = TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int);
+  if (n > E->qcap) {
+    if (E->qws) {
+      auto it = std::find(E->allocs.begin(), E->allocs.end(), (void *)E->qws);
+      if (it != E->allocs.end()) E->allocs.erase(it);
+      HIP_OK(hipFree(E->qws));
+      E->qws = nullptr;
+      E->qcap = 0;
+    }
+    const uint64_t cap = std::max<uint64_t>(n, 1024);
+    E->qws = reinterpret_cast<char *>(E->dalloc(per_traj * cap + sizeof(unsigned long long) * SLOT_SLACK));
+    E->qcap = cap;
+  }
+  Queue Q;
+  Q.ticket = E->qctl + 0 * QLINE;
+  Q.shead = E->qctl + 1 * QLINE;
+  Q.stail = E->qctl + 2 * QLINE;
+  Q.active = reinterpret_cast<long long *>(E->qctl + 3 * QLINE);
+  Q.work = E->qctl + 4 * QLINE;
+  Q.stalled = reinterpret_cast<int *>(E->qctl + 5 * QLINE);
+  Q.tab = reinterpret_cast<TryOut *>(E->qws);
+  Q.slots = reinterpret_cast<unsigned long long *>(E->qws + TRIES_MAX * sizeof(TryOut) * n);
+  Q.nslots = 8 * n + SLOT_SLACK;
+  Q.tries = reinterpret_cast<int *>(Q.slots + Q.nslots);
+  Q.spawned = Q.tries + n;
+  Q.resolved = Q.spawned + n;
+  Q.wait_ticks = E->wait_ticks;
+  // automatic: spawning starts once fewer trajectories than resident groups
+  // are unresolved, i.e. when groups begin to fall idle
+  Q.spec_below = E->spec_below >= 0 ? E->spec_below : (long long)(blocks * groups_per_block);
+  hipLaunchKernelGGL(queue_init, dim3(256), dim3(256), 0, s, Q, n);
+  HIP_OK(hipGetLastError());
   Tgt T = E->T;
-  unsigned long long *q = E->queue;
-  void *args[] = {&T, &u_dev, &n, &rec_dev, &q};
+  void *args[] = {&T, &u_dev, &n, &rec_dev, &Q};
   HIP_OK(hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(E->block), args, E->shmem, s));
+  E->last_stream = s;
+}
+
+// Throws when a group of the last run gave up waiting for work (its records
+// are then incomplete).
+static void check_stall(Engine *E) {
+  unsigned long long c = 0;
+  HIP_OK(hipMemcpy(&c, E->qctl + 5 * QLINE, sizeof c, hipMemcpyDeviceToHost));
+  if ((int)(c & 0xffffffffu) != 0)
+    throw Error(MCCS_ERR_DEVICE, "trajectory work queue stalled; records are incomplete");
+}
+
+void engine_last_work(Engine *E, uint64_t out[5]) {
+  HIP_OK(hipSetDevice(E->device));
+  HIP_OK(hipStreamSynchronize(E->last_stream ? E->last_stream : E->stream));
+  check_stall(E);
+  HIP_OK(hipMemcpy(out, E->qctl + 4 * QLINE, sizeof(uint64_t) * 5, hipMemcpyDeviceToHost));
 }
 
 void engine_run_host(Engine *E, const double *u, uint64_t n, mccs_traj_record *rec) {
@@ -305,6 +371,7 @@ void engine_run_host(Engine *E, const double *u, uint64_t n, mccs_traj_record *r
     engine_run_device(E, du, n, dr, E->stream);
     HIP_OK(hipMemcpyAsync(rec, dr, sizeof(mccs_traj_record) * n, hipMemcpyDeviceToHost, E->stream));
     HIP_OK(hipStreamSynchronize(E->stream));
+    check_stall(E);
   } catch (...) {
     hipFree(du);
     hipFree(dr);

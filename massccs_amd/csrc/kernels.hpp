@@ -49,7 +49,7 @@ constexpr double KE_CONV = 2390.07;                          // amuAngfs2_to_KCA
 constexpr double KC = 332.06348078020324;                    // KCOUL
 constexpr double PI = 3.14159265358979323846;
 constexpr double FAR = 1.0e30;                               // parks an inactive site
-constexpr int MAX_STEPS = 4000000;                           // per trajectory, over all tries
+constexpr int MAX_STEPS = 4000000;                           // per try
 constexpr double SKIN = 1.0;                                 // Verlet-list skin (A)
 
 enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4 };
@@ -192,6 +192,14 @@ __device__ __forceinline__ void group_ballot(bool flag, int lane, int &below, in
   const unsigned long long gm = (L == 64) ? m : ((m >> gbase) & ((1ull << L) - 1ull));
   total = __popcll(gm);
   below = __popcll(gm & ((1ull << lane) - 1ull));
+}
+
+// Bits of a wave ballot that belong to this lane's group, bit i = group lane i.
+template <int L>
+__device__ __forceinline__ unsigned long long group_mask(bool flag, int lane) {
+  const unsigned long long m = __ballot(flag);
+  const int gbase = (int)(threadIdx.x % 64) - lane;
+  return (L == 64) ? m : ((m >> gbase) & ((1ull << L) - 1ull));
 }
 
 __device__ __forceinline__ void lds_fence() {
@@ -869,9 +877,84 @@ struct GState {
   double Ei, dt, dtheta, hdm, Utot;
   Rat rc;
   VList vl;
-  unsigned long long j;
-  int status, tries, max_tries, steps;
+  unsigned long long j;         // trajectory
+  int k;                        // try (restart) index of the trajectory, 0..TRIES_MAX-1
+  unsigned long long slot;      // spawn slot taken while its item is not yet written
+  int steps;                    // steps of this try
+  int cont;                     // 1: the group goes on with try k of trajectory j
+  int has_slot;
 };
+
+// Outcome of one try of one trajectory.
+enum : int { TRY_PENDING = 0, TRY_SUCCESS = 1, TRY_ANGLE = 2, TRY_ENERGY = 3, TRY_CAPPED = 4,
+             TRY_ABANDON = 5 };  // TRY_ABANDON: moot, never stored
+// The reference's retry loop (src/System.cpp:392, maxTries = 8 raised by one
+// per energy failure while below 10) runs at most 9 tries: e.g. seven
+// angular restarts, an energy failure, and a ninth try.
+constexpr int TRIES_MAX = 9;
+struct TryOut {
+  int code, steps;
+  double chi;
+  uint32_t visits, lj, coul, dip;
+};
+
+// Device work queue.  Primary items are tries 0 of trajectories [0, n) in
+// index order (ticket counter).  Each try k restarts from the trajectory's
+// initial state with dt0 / (k + 1), exactly as the reference's loop does after
+// k failed tries (src/System.cpp:453-458, :466-473), so the tries of one
+// trajectory are independent of each other and may run concurrently; the
+// reference's sequence of restarts is then folded over their outcomes in order.
+// tries[j] counts the tries of j claimed so far (each index is claimed once).
+// A group whose try fails claims the trajectory's next try and goes on with
+// it.  Once fewer than spec_below trajectories are unresolved (the end of the
+// run, when groups fall idle), the group running a trajectory that has
+// already restarted, or whose first try has run long, claims all its
+// remaining tries at once and spawns them as items that idle groups take, so
+// the last trajectories' tries run side by side instead of one after another.
+// A claimed try that the fold turns out not to need is abandoned at its next
+// check.
+struct Queue {
+  unsigned long long *ticket;  // primary tickets handed out
+  unsigned long long *shead;   // spawned items taken
+  unsigned long long *stail;   // spawned items reserved
+  long long *active;           // trajectories not yet resolved
+  unsigned long long *work;    // executed totals: visits, lj, coul, dip, steps
+  int *stalled;                // set by a group that gave up waiting
+  unsigned long long *slots;   // spawned items j * 16 + k, ~0 until written
+  unsigned long long nslots;
+  int *tries;                  // tries claimed, per trajectory
+  int *spawned;                // 1 once the trajectory's remaining tries are spawned
+  int *resolved;               // 1 once the trajectory's record is written
+  TryOut *tab;                 // TRIES_MAX per trajectory
+  unsigned long long wait_ticks;  // wall-clock ticks a group waits for work at most
+  long long spec_below;        // spawning starts below this many unresolved trajectories
+};
+
+// A first try that has run this many steps is likely to orbit and restart.
+constexpr int SPAWN_STEPS = 16384;
+
+__device__ __forceinline__ unsigned long long ld_acq(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_acq_i(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T_>
+__device__ __forceinline__ T_ ld_rlx(const T_ *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Claims every try of trajectory j not claimed yet and spawns them (lane 0).
+__device__ void spawn_rest(const Queue &Q, uint64_t j) {
+  if (atomicCAS(Q.spawned + j, 0, 1) != 0) return;
+  const int c = atomicExch(Q.tries + j, TRIES_MAX);
+  if (c >= TRIES_MAX) return;
+  const unsigned long long r0 = atomicAdd(Q.stail, (unsigned long long)(TRIES_MAX - c));
+  for (int kk = c; kk < TRIES_MAX; kk++)
+    if (r0 + (kk - c) < Q.nslots)
+      __hip_atomic_store(Q.slots + r0 + (kk - c), (unsigned long long)(16 * j + kk),
+                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Block LDS layout: atoms and LJ pairs (when staged) | per-lane run scratch
 // (2 ints per thread) | one GState per group | one Verlet list per group.
@@ -920,14 +1003,75 @@ __device__ void stage(const Tgt &T, Shm &sh, char *smem) {
 }
 
 // ---------------------------------------------------------------- kernels
-// Persistent trajectory kernel.  Each group of L lanes pulls trajectory
-// indices from *queue until n is reached; per loop iteration every live
-// group performs exactly one force evaluation: the initial force of a new
-// (or restarted) trajectory, or the force of one velocity-Verlet/RATTLE step.
+template <int L>
+__device__ __forceinline__ unsigned long long bcast(unsigned long long v) {
+  return __shfl(v, 0, L);
+}
+
+// Folds the tries of trajectory j in order, as the reference's retry loop
+// would have run them (src/System.cpp:389-511): a try that succeeds ends the
+// trajectory; an angular restart (dtheta > 1.5 pi) counts a try, an energy
+// failure counts a try and raises the allowance; the loop stops when
+// tries >= maxTries or maxTries reaches 10.  Writes the record once the
+// outcome is determined (by whichever completing try determines it first).
+// Returns true when the trajectory is resolved.  Called by lane 0 after the
+// caller's own outcome is stored; the sequentially consistent fence makes sure
+// that of two tries ending together at least one sees the other's outcome.
+__device__ bool resolve(const Queue &Q, mccs_traj_record *rec, uint64_t j, double bmax) {
+  __atomic_thread_fence(__ATOMIC_SEQ_CST);
+  int tries = 0, max_tries = 8, steps = 0, success = 0, final = -1, capped = 0;
+  uint32_t v = 0, l = 0, c = 0, d = 0;
+  double chi = 0.0;
+  for (int k = 0; k < TRIES_MAX; k++) {
+    const int code = ld_acq_i(&Q.tab[TRIES_MAX * j + k].code);
+    if (code == TRY_PENDING) return ld_acq_i(&Q.resolved[j]) != 0;  // not determined yet
+    const TryOut t = Q.tab[TRIES_MAX * j + k];
+    steps += t.steps;
+    v += t.visits; l += t.lj; c += t.coul; d += t.dip;
+    if (code == TRY_SUCCESS) {
+      success = 1;
+      chi = t.chi;
+      final = k;
+      break;
+    }
+    if (code == TRY_CAPPED) {
+      capped = 1;
+      final = k;
+      break;
+    }
+    tries = k + 1;
+    if (code == TRY_ENERGY) max_tries++;
+    if (!(tries < max_tries && max_tries < 10) || k == TRIES_MAX - 1) {
+      final = k;
+      break;
+    }
+  }
+  if (atomicCAS(&Q.resolved[j], 0, 1) != 0) return true;
+  mccs_traj_record r;
+  r.hit = 1;
+  r.success = success;
+  r.tries = capped ? -1 : (success ? final : tries);
+  r.steps = steps;
+  r.chi = chi;
+  r.domega = success ? PI * (1.0 - cos(chi)) * (bmax * bmax) : 0.0;
+  r.visits = v;
+  r.lj = l;
+  r.coul = c;
+  r.dip = d;
+  rec[j] = r;
+  __threadfence();
+  atomicAdd((unsigned long long *)Q.active, ~0ull);  // -1
+  return true;
+}
+
+// Persistent trajectory kernel.  Each group of L lanes pulls work items
+// (trajectory, try) until every trajectory is resolved; per loop iteration
+// every working group performs exactly one force evaluation: the initial
+// force of a try, or the force of one velocity-Verlet/RATTLE step.
 template <int GK, int FT, int L, bool LDS, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void traj_kernel(
     Tgt T, const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec,
-    unsigned long long *__restrict__ queue) {
+    Queue Q) {
   using S = Spec<GK, FT>;
   constexpr bool CEN = any_central<S>();
   extern __shared__ __align__(16) char smem[];
@@ -945,37 +1089,121 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 
   Work w{0, 0, 0, 0};
   int status = ST_NEED;
+  unsigned long long wait_since = 0;  // wall clock when the group began waiting, 0 if not
   if (lane == 0) {
-    st.status = ST_NEED;
     st.vl.valid = -1;
+    st.cont = 0;
+    st.has_slot = 0;
   }
   lds_fence();
 
   while (status != ST_DONE) {
-    // ---- phase A: new trajectory, restart, or first half-step
-    if (status == ST_NEED || status == ST_RESTART) {
+    // ---- phase A: take a work item, or the first half-step
+    if (status == ST_NEED) {
       double pos[3][3], vel[3][3], rcm[3], vcm[3];
-      uint64_t j = st.j;
-      bool ok = false;
-      if (status == ST_RESTART) {
-        setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);
-        ok = true;
+      uint64_t j = 0;
+      int k = 0;
+      bool got = false, wait = false, done = false;
+      if (st.cont) {  // the next try of the trajectory whose try just failed
+        j = st.j;
+        k = st.k;
+        setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
+        got = true;
       }
-      while (!ok) {
+      while (!got && !wait && !done) {
+        // 1. the next primary trajectory (the loop only repeats past misses)
         unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(queue, 1ull);
-        t = __shfl(t, 0, L);
-        if (t >= n) break;
-        j = t;
-        w = Work{0, 0, 0, 0};
-        if (setup<S>(T, u + 9 * j, pos, vel, rcm, vcm)) ok = true;
-        else write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, T.bmax, w, COUNT);
+        if (lane == 0) {  // look before taking: idle groups must not hammer the counter
+          t = __hip_atomic_load(Q.ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (t < n) t = atomicAdd(Q.ticket, 1ull);
+        }
+        t = bcast<L>(t);
+        if (t < n) {
+          j = t;
+          k = 0;
+          w = Work{0, 0, 0, 0};
+          if (setup<S>(T, u + 9 * j, pos, vel, rcm, vcm)) {
+            got = true;
+          } else {
+            write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, T.bmax, w, COUNT);
+            if (lane == 0) {
+              __threadfence();
+              atomicAdd((unsigned long long *)Q.active, ~0ull);
+            }
+          }
+          continue;
+        }
+        // 2. a spawned try (the slot taken earlier, or the next one)
+        if (Q.spec_below <= 0) {  // nothing is ever spawned: this group is finished
+          done = true;
+          break;
+        }
+        unsigned long long slot = ~0ull;
+        if (lane == 0) {
+          if (st.has_slot) {
+            slot = st.slot;
+          } else if (ld_rlx(Q.shead) < ld_rlx(Q.stail)) {
+            slot = atomicAdd(Q.shead, 1ull);
+          }
+        }
+        slot = bcast<L>(slot);
+        if (slot != ~0ull) {
+          unsigned long long item = ~0ull;
+          int moot = 0;
+          if (lane == 0) {
+            item = slot < Q.nslots ? ld_acq(Q.slots + slot) : ~0ull;
+            st.has_slot = item == ~0ull;  // written soon by its spawner, or never
+            st.slot = slot;
+            if (item != ~0ull) moot = ld_rlx(Q.resolved + (item >> 4));
+          }
+          item = bcast<L>(item);
+          moot = __shfl(moot, 0, L);
+          if (item != ~0ull) {
+            if (moot) continue;  // the trajectory is resolved: take the next one
+            j = item >> 4;
+            k = (int)(item & 15);
+            setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
+            got = true;
+            break;
+          }
+        }
+        // 3. nothing to take: finished once every trajectory is resolved
+        long long act = 0;
+        if (lane == 0) act = __hip_atomic_load(Q.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        act = (long long)bcast<L>((unsigned long long)act);
+        if (act <= 0) done = true;
+        else wait = true;
       }
-      if (!ok) {
+      if (wait) {
+        // a try in flight always ends (MAX_STEPS), so waiting ends; a group
+        // that has waited far longer than any try takes gives up and flags it
+        const unsigned long long now = wall_clock64();
+        if (!wait_since) wait_since = now;
+        if (now - wait_since > Q.wait_ticks) {
+          if (lane == 0) __hip_atomic_store(Q.stalled, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          done = true;
+        }
+      }
+      if (done) {
         status = ST_DONE;
         break;
       }
+      if (wait) {
+        // a whole-wave group sleeps ~24k cycles between looks; a smaller group
+        // shares its wave with working groups and looks once per their step
+        if (L == 64) {
+          __builtin_amdgcn_s_sleep(127);
+          __builtin_amdgcn_s_sleep(127);
+          __builtin_amdgcn_s_sleep(127);
+        }
+        lds_fence();
+        continue;
+      }
+      wait_since = 0;
+      w = Work{0, 0, 0, 0};
       if (lane == 0) {
+        st.vl.valid = -1;  // each try builds its own neighbour list: results do not
+                           // depend on what the group ran before
         for (int s = 0; s < S::NS; s++)
           for (int d = 0; d < 3; d++) {
             st.pos[s][d] = pos[s][d];
@@ -986,8 +1214,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
           st.vcm[d] = vcm[d];
         }
         st.j = j;
+        st.k = k;
+        st.cont = 0;
       }
-      status = status == ST_RESTART ? ST_RESTART : ST_INIT;
+      status = ST_INIT;
     } else {  // ST_STEP: first half
       const double dt = st.dt;
       double rcm[3];
@@ -1042,7 +1272,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     }
     lds_fence();
 
-    // ---- phase B: one force evaluation for every live group
+    // ---- phase B: one force evaluation for every working group
     {
       double p[S::NS][3], f[S::NS][3], U[S::NS];
 #pragma unroll
@@ -1061,24 +1291,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     }
     lds_fence();
 
-    // ---- phase C: bookkeeping, second half-step, exit tests
-    if (status == ST_INIT || status == ST_RESTART) {
+    // ---- phase C: try initialisation, or second half-step and exit tests
+    if (status == ST_INIT) {
       if (lane == 0) {
-        if (status == ST_INIT) {
-          double Ek = 0.0;
-          if (S::NS == 1) Ek = kinetic(T.mu, st.vel[0]);
-          else  // the N2 dummy is massless and adds nothing
-            for (int s = 0; s < (GK == 2 ? 2 : 3); s++) Ek += kinetic(S::m(s), st.vel[s]);
-          st.Ei = st.Utot + Ek;
-          st.tries = 0;
-          st.max_tries = 8;
-          st.steps = 0;
-          st.dt = T.dt;
-          st.hdm = 0.5 * T.dt / T.mu;
-          st.rc = rattle_coeffs(MA, MB, T.dt);
-          for (int d = 0; d < 3; d++) st.vcm_i[d] = st.vcm[d];
-          if (GK == 3) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
-        }
+        double Ek = 0.0;
+        if (S::NS == 1) Ek = kinetic(T.mu, st.vel[0]);
+        else  // the N2 dummy is massless and adds nothing
+          for (int s = 0; s < (GK == 2 ? 2 : 3); s++) Ek += kinetic(S::m(s), st.vel[s]);
+        st.Ei = st.Utot + Ek;
+        st.steps = 0;
+        const double dt = T.dt / (st.k + 1);  // dt of try k, src/System.cpp:455
+        st.dt = dt;
+        st.hdm = 0.5 * dt / T.mu;
+        st.rc = rattle_coeffs(MA, MB, dt);
+        for (int d = 0; d < 3; d++) st.vcm_i[d] = st.vcm[d];
+        if (GK == 3) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
         st.dtheta = 0.0;
         for (int d = 0; d < 3; d++) st.rold[d] = st.rcm[d];
       }
@@ -1121,18 +1348,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     double rcm[3];
     for (int d = 0; d < 3; d++) rcm[d] = st.rcm[d];
     const int steps = st.steps + 1;
-    int tries = st.tries, max_tries = st.max_tries;
-    bool restart = false, finished = false;
-    int success = 0;
+    const uint64_t j = st.j;
+    const int k = st.k;
+    int code = TRY_PENDING;
     double chi = 0.0;
     if (steps >= MAX_STEPS || !isfinite(rcm[0] + rcm[1] + rcm[2] + vcm[0] + vcm[1] + vcm[2])) {
       // the reference would loop forever here (zero speed or non-finite
       // state); the trajectory is reported lost with tries = -1
-      finished = true;
-      tries = -1;
+      code = TRY_CAPPED;
     } else if (st.dtheta > 1.5 * PI) {
-      tries++;
-      restart = true;
+      code = TRY_ANGLE;
     } else if (fabs(rcm[0]) >= T.hx || fabs(rcm[1]) >= T.hy || fabs(rcm[2]) >= T.hz ||
                (rcm[0] * iea) * (rcm[0] * iea) + (rcm[1] * ieb) * (rcm[1] * ieb) +
                        (rcm[2] * iec) * (rcm[2] * iec) >
@@ -1142,42 +1367,121 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
                                    : fabs((Ut + Ek - Ei) / Ei) * 100.0;
       const double tol = GK == 1 ? 0.5 : (GK == 2 ? 1.0 : 20.0);
       if (dE > tol) {
-        tries++;
-        max_tries++;
-        restart = true;
+        code = TRY_ENERGY;
       } else {
         double vi[3];
         for (int d = 0; d < 3; d++) vi[d] = st.vcm_i[d];
         chi = anglevec(vi, vcm);
-        success = 1;
-        finished = true;
+        code = TRY_SUCCESS;
       }
-    }
-    if (restart && !(tries < max_tries && max_tries < 10)) finished = true;  // lost
-    const uint64_t j = st.j;
-    lds_fence();
-    if (finished) {
-      write_record<L>(rec, j, lane, 1, success, tries, steps, chi, T.bmax, w, COUNT);
-      status = ST_NEED;
-    } else if (restart) {
-      const double dt = T.dt / (tries + 1);
+    } else if ((steps & 255) == 0) {
+      // a later try is moot once the trajectory is resolved or an earlier
+      // try has succeeded; near the end of the run, a trajectory that has
+      // restarted or runs long spawns its remaining tries
+      int moot = 0;
       if (lane == 0) {
-        st.tries = tries;
-        st.max_tries = max_tries;
-        st.steps = steps;
-        st.dt = dt;
-        st.hdm = 0.5 * dt / T.mu;
-        st.rc = rattle_coeffs(MA, MB, dt);
+        if (k > 0) {
+          moot = ld_acq_i(&Q.resolved[j]);
+          for (int kk = 0; kk < k && !moot; kk++)
+            moot = ld_acq_i(&Q.tab[TRIES_MAX * j + kk].code) == TRY_SUCCESS;
+        }
+        if (!moot && (k > 0 || steps >= SPAWN_STEPS) && ld_rlx(Q.spawned + j) == 0 &&
+            ld_rlx(Q.active) < Q.spec_below)
+          spawn_rest(Q, j);
       }
-      status = ST_RESTART;
-    } else {
-      if (lane == 0) st.steps = steps;
+      if (__shfl(moot, 0, L)) code = TRY_ABANDON;
     }
+    lds_fence();
+    if (code == TRY_PENDING) {
+      if (lane == 0) st.steps = steps;
+      lds_fence();
+      continue;
+    }
+    // the try has ended
+    uint32_t wv = 0, wl = 0, wc = 0, wd = 0;
+    if (COUNT) {
+      wv = gsum_u<L>(w.visits);
+      wl = gsum_u<L>(w.lj);
+      wc = gsum_u<L>(w.coul);
+      wd = gsum_u<L>(w.dip);
+    }
+    if (lane == 0) {
+      if (COUNT) {
+        atomicAdd(Q.work + 0, (unsigned long long)wv);
+        atomicAdd(Q.work + 1, (unsigned long long)wl);
+        atomicAdd(Q.work + 2, (unsigned long long)wc);
+        atomicAdd(Q.work + 3, (unsigned long long)wd);
+      }
+      atomicAdd(Q.work + 4, (unsigned long long)steps);
+      if (k == 0 && (code == TRY_SUCCESS || code == TRY_CAPPED) && ld_rlx(Q.spawned + j) == 0) {
+        // the common case: the first try decides (no retry bookkeeping; only
+        // this group can have spawned later tries of j while try 0 ran)
+        mccs_traj_record r;
+        r.hit = 1;
+        r.success = code == TRY_SUCCESS;
+        r.tries = code == TRY_SUCCESS ? 0 : -1;
+        r.steps = steps;
+        r.chi = chi;
+        r.domega = code == TRY_SUCCESS ? PI * (1.0 - cos(chi)) * (T.bmax * T.bmax) : 0.0;
+        r.visits = wv;
+        r.lj = wl;
+        r.coul = wc;
+        r.dip = wd;
+        rec[j] = r;
+        __threadfence();
+        atomicAdd((unsigned long long *)Q.active, ~0ull);
+      } else if (code != TRY_ABANDON) {
+        TryOut o;
+        o.code = TRY_PENDING;
+        o.steps = steps;
+        o.chi = chi;
+        o.visits = wv;
+        o.lj = wl;
+        o.coul = wc;
+        o.dip = wd;
+        Q.tab[TRIES_MAX * j + k] = o;
+        __threadfence();
+        __hip_atomic_store(&Q.tab[TRIES_MAX * j + k].code, code, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // a failed try needs the next one unless the fold is already decided;
+        // claim it unless another group has, and near the end of the run
+        // spawn the ones after it
+        if (!resolve(Q, rec, j, T.bmax) && code != TRY_SUCCESS && code != TRY_CAPPED &&
+            k + 1 < TRIES_MAX) {
+          if (atomicCAS(Q.tries + j, k + 1, k + 2) == k + 1) {
+            st.k = k + 1;
+            st.cont = 1;
+          }
+          if (ld_rlx(Q.active) < Q.spec_below) spawn_rest(Q, j);
+        }
+      }
+    }
+    status = ST_NEED;
     lds_fence();
   }
 }
 
-using TrajFn = void (*)(Tgt, const double *, uint64_t, mccs_traj_record *, unsigned long long *);
+static __global__ void queue_init(Queue Q, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t s = i; s < n; s += stride) {
+    Q.tries[s] = 1;  // try 0 belongs to the primary ticket
+    Q.spawned[s] = 0;
+    Q.resolved[s] = 0;
+  }
+  for (uint64_t s = i; s < Q.nslots; s += stride) Q.slots[s] = ~0ull;
+  for (uint64_t s = i; s < TRIES_MAX * n; s += stride) Q.tab[s].code = TRY_PENDING;
+  if (i == 0) {
+    *Q.ticket = 0;
+    *Q.shead = 0;
+    *Q.stail = 0;
+    *Q.active = (long long)n;
+    for (int k = 0; k < 5; k++) Q.work[k] = 0;
+    *Q.stalled = 0;
+  }
+}
+
+using TrajFn = void (*)(Tgt, const double *, uint64_t, mccs_traj_record *, Queue);
 
 template <int GK, int FT, int L>
 TrajFn pick_traj(bool lds, bool count) {

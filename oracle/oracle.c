@@ -734,10 +734,10 @@ static inline double ellipsoid_r(const oracle_params *p, const double r[3]) {
 
 /* The reference loops forever for a probe that never leaves (zero speed or a
  * non-finite state); here such a trajectory is reported lost with tries = -1
- * after this many steps, as the engine does. */
+ * once one try has run this many steps, as the engine does. */
 #define ORACLE_MAX_STEPS 4000000
-#define CAPPED(o, r, v)                                                        \
-  ((o)->steps >= ORACLE_MAX_STEPS ||                                            \
+#define CAPPED(sc, r, v)                                                       \
+  ((sc) >= ORACLE_MAX_STEPS ||                                                  \
    !isfinite((r)[0] + (r)[1] + (r)[2] + (v)[0] + (v)[1] + (v)[2]))
 
 /* System::run_He, src/System.cpp:338-511 (also Ar and 1-site co2) */
@@ -778,7 +778,7 @@ static void run_atomic(const oracle_ctx *c, const double pos0[3],
     for (int i = 0; i < 3; i++) vcm[i] += 0.5 * dt / mu * f[i] * KCALMOLANGAMU_TO_ANGFS2;
     stepcount++;
     o->steps++;
-    if (CAPPED(o, rcm, vcm)) {
+    if (CAPPED(stepcount, rcm, vcm)) {
       o->success = 0;
       o->tries = -1;
       return;
@@ -889,7 +889,7 @@ static void run_n2(const oracle_ctx *c, const double pos0[9], const double vel0[
     }
     stepcount++;
     o->steps++;
-    if (CAPPED(o, rcm, vcm)) {
+    if (CAPPED(stepcount, rcm, vcm)) {
       o->success = 0;
       o->tries = -1;
       return;
@@ -1010,7 +1010,7 @@ static void run_co2(const oracle_ctx *c, const double pos0[9], const double vel0
     }
     stepcount++;
     o->steps++;
-    if (CAPPED(o, rcm, vcm)) {
+    if (CAPPED(stepcount, rcm, vcm)) {
       o->success = 0;
       o->tries = -1;
       return;

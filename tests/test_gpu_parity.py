@@ -174,3 +174,32 @@ def test_tally_device_matches_host():
     assert np.array_equal(d_om.cpu().numpy(), om)
     assert np.array_equal(d_cnt.cpu().numpy().reshape(3, 3), cnt)
     assert abs(d_me[0].item() - mean) <= 1e-12 * abs(mean)
+
+
+def test_restarts_run_side_by_side_with_identical_results(workdir):
+    """Later tries of a restarting trajectory may run concurrently on idle
+    groups; the fold over their outcomes must give exactly the records of
+    running them one after another, whatever the schedule (each try builds
+    its own neighbour list, so even the rounding is schedule-independent)."""
+    cfg = globule_cfg(workdir)
+    print("case globule_N2 restarts", flush=True)
+    P = prepared(cfg)
+    u = P.samples(4000)
+    runs = []
+    for spec in (0, -1, 10 ** 9):
+        E = M.Engine(P, spec_below=spec)
+        runs.append(E.run(u))
+        w = E.last_work()
+        assert w["steps"] >= runs[-1]["steps"].sum()
+        E.close()
+    for r in runs[1:]:
+        assert np.array_equal(r.view(np.uint8), runs[0].view(np.uint8))
+    r = runs[0]
+    assert (r["tries"] >= 4).sum() >= 5  # the sample holds long restart chains
+    # the three longest chains against the oracle: same restarts and fate; the
+    # orbits are chaotic, so rounding shifts their exit by a few steps
+    idx = np.argsort(r["steps"])[-3:]
+    ref = oracle_for(P, cfg).run(u[idx])
+    assert np.array_equal(ref["tries"], r["tries"][idx])
+    assert np.array_equal(ref["success"], r["success"][idx])
+    assert np.all(np.abs(ref["steps"] - r["steps"][idx].astype(np.int64)) <= 0.01 * ref["steps"])
