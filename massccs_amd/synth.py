@@ -4,7 +4,8 @@
 * globule(): random sequential addition of atoms of a fixed composition in an
              ellipsoid with a minimum spacing, charges U[-0.5, 0.5] shifted to
              a chosen net charge (the 1,231-atom ubiquitin-size case, and the
-             50,000-atom ellipsoid)
+             50,000-atom ellipsoid; above 20,000 atoms candidates are added
+             in batches)
 * shell():   the same inside a hollow sphere (1,000,000-atom case)
 
 All write the reference's .xyz format (src/MoleculeTarget.cpp:148-319):
@@ -132,6 +133,34 @@ def _rsa(n, inside, sample_box, min_dist, rng, batch=4096):
     return pts
 
 
+def _rsa_batched(n, inside, sample_box, min_dist, rng):
+    """Random addition in batches for large n: a batch of candidates is
+    thinned against the accepted points and against earlier candidates of the
+    same batch (k-d trees), so every accepted pair is at least min_dist apart."""
+    from scipy.spatial import cKDTree
+    lo, hi = sample_box
+    pts = np.zeros((0, 3))
+    tree = None
+    while len(pts) < n:
+        m = max(20000, len(pts) // 4)
+        cand = rng.uniform(lo, hi, size=(m, 3))
+        cand = cand[inside(cand)]
+        if tree is not None and len(cand):
+            d, _ = tree.query(cand, k=1, distance_upper_bound=min_dist)
+            cand = cand[~(d < min_dist)]
+        if len(cand) > 1:
+            pairs = cKDTree(cand).query_pairs(min_dist, output_type="ndarray")
+            keep = np.ones(len(cand), dtype=bool)
+            keep[pairs.max(axis=1)] = False  # the later candidate of each close pair
+            cand = cand[keep]
+        cand = cand[: n - len(pts)]
+        if len(cand) == 0:
+            raise RuntimeError("random addition did not converge")
+        pts = np.concatenate([pts, cand])
+        tree = cKDTree(pts)
+    return pts
+
+
 def _composition(n, rng, fractions=GLOBULE_FRACTIONS):
     counts = np.floor(np.array(fractions) * n).astype(int)
     counts[np.argmax(fractions)] += n - counts.sum()
@@ -151,7 +180,7 @@ def globule(n=1231, axes=(18.0, 15.0, 13.0), net_charge=4.0, min_dist=1.0, seed=
     rng = np.random.default_rng(seed)
     a = np.array(axes, dtype=np.float64)
     inside = lambda P: ((P / a) ** 2).sum(axis=1) <= 1.0
-    xyz = _rsa(n, inside, (-a, a), min_dist, rng)
+    xyz = (_rsa if n <= 20000 else _rsa_batched)(n, inside, (-a, a), min_dist, rng)
     return _composition(n, rng), xyz, _charges(n, net_charge, rng)
 
 
@@ -159,7 +188,8 @@ def shell(n=1_000_000, r_in=120.0, r_out=140.0, net_charge=60.0, min_dist=1.0, s
     """Atoms packed in a hollow sphere; returns (symbols, xyz, q)."""
     rng = np.random.default_rng(seed)
     inside = lambda P: (lambda r: (r >= r_in) & (r <= r_out))(np.linalg.norm(P, axis=1))
-    xyz = _rsa(n, inside, (-np.full(3, r_out), np.full(3, r_out)), min_dist, rng)
+    box = (-np.full(3, r_out), np.full(3, r_out))
+    xyz = (_rsa if n <= 20000 else _rsa_batched)(n, inside, box, min_dist, rng)
     return _composition(n, rng), xyz, _charges(n, net_charge, rng)
 
 

@@ -10,7 +10,7 @@
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
  * checker; the product never links or calls it.
  *
- * Pinning (see tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle.py):
  *   - per-point site forces of every kernel against the reference harness
  *     (oracle/_ref/ref_harness force) and the committed fixtures;
  *   - per-trajectory (hit, success, chi) against the reference harness;
