@@ -105,6 +105,130 @@ def cpu_baseline(cfg, budget_s=15.0):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def big_target_cfg(which):
+    """In-memory synthetic targets of the large configurations (N2, type 6)."""
+    from massccs_amd import synth
+    if which == "shell1m":
+        s, xyz, q = synth.shell()
+        label = "1,000,000-atom hollow shell r 120-140 A, net +60 e"
+    else:
+        s, xyz, q = synth.globule(n=50000, axes=(60.0, 45.0, 35.0), net_charge=20.0, seed=2104)
+        label = "50,000-atom ellipsoid 60/45/35 A, net +20 e"
+    cfg = {"targetFileName": "memory.xyz", "GasBuffer": "N2", "Short-range cutoff": "yes",
+           "LJ-cutoff": 12.0, "Long-range forces": "yes", "Long-range cutoff": "yes",
+           "Coul-cutoff": 25.0, "polarizability": "yes", "alpha": 1.7403, "seed": 2104,
+           "dt": 10.0, "Temp": 298.0}
+    return cfg, (s, xyz, q), label
+
+
+def run_big(args, which):
+    """Plain FP64 engine on one large target (samples resident in HBM)."""
+    import torch
+    import massccs_amd as M
+    cfg, (s, xyz, q), label = big_target_cfg(which)
+    n = args.traj or (10 ** 6 if which == "shell1m" else 10 ** 5)
+    P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
+    E = M.Engine(P, count_work=True)
+    u = torch.from_numpy(P.samples(n)).cuda()
+    rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    for _ in range(args.warmup):
+        E.run_device(u.data_ptr(), min(n, 20000), rec.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    for _ in range(args.steps):
+        e0.record(stream)
+        E.run_device(u.data_ptr(), n, rec.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    om, cnt, mean, err = M.tally(r, 1, n)
+    flops = _flops(r)
+    kern_ms = e0.elapsed_time(e1)
+    return {
+        "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "trajectories/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded target and mt19937 samples)",
+        "config": {"workload": "plain FP64, %s + N2, force type 6, %d trajectories" % (label, n),
+                   "n_atoms": P.info.natoms, "trajectories_per_step": n, "probe": "N2",
+                   "force_type": P.info.force_type, "parallelism": "1 GPU"},
+        "force_evals_per_s": round(3 * int(r["steps"].sum()) / (kern_ms * 1e-3), 1),
+        "ccs": {"mean": mean, "lost": int(((r["hit"] == 1) & (r["success"] == 0)).sum())},
+        "roofline": {"bound": "mfma", "achieved": round(flops / (kern_ms * 1e-3) / 1e12, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "traj_kernel<N2, type 6>",
+                     "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
+        "cpu_baseline": None,
+    }
+
+
+def _flops(r):
+    hits = r["hit"] == 1
+    evals = (1 + r["steps"] + np.maximum(r["tries"], 0)) * hits
+    return float(W_VISIT * r["visits"].sum() + W_LJ * r["lj"].sum() + W_COUL * r["coul"].sum()
+                 + W_DIP * r["dip"].sum() + W_EPI * evals.sum())
+
+
+def run_batch(args):
+    """The 64-assembly batch: one persistent launch per probe kind, autotuned
+    launch shapes, replayed from a hipGraph."""
+    import massccs_amd as M
+    from massccs_amd import synth
+    descs = synth.assemblies()
+    per = args.traj or 20000
+    P, us = [], []
+    for d in descs:
+        s, xyz, q = synth.globule(n=d["n"], axes=d["axes"], net_charge=d["net_charge"],
+                                  seed=d["seed"])
+        cfg = {"targetFileName": "memory.xyz", "GasBuffer": d["probe"], "polarizability": "yes",
+               "Short-range cutoff": "yes", "Long-range forces": "yes",
+               "Long-range cutoff": "yes", "seed": d["seed"]}
+        if d["probe"] == "N2":
+            cfg["alpha"] = 1.7403
+        p = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
+        P.append(p)
+        us.append(p.samples(per))
+    B = M.Batch(P)
+    t0 = time.perf_counter()
+    tune = B.autotune([u[: max(64, per // 16)] for u in us])
+    t_tune = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        B.run(us)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        recs = B.run(us)
+    wall = time.perf_counter() - t0
+    n = per * len(P)
+    r = np.concatenate(recs)
+    steps = np.array([int(x["steps"].sum()) for x in recs])
+    return {
+        "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "trajectories/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (64 seeded assemblies and mt19937 samples)",
+        "config": {"workload": "batched FP64, 64 assemblies of 514-199,788 atoms (He type 4 / "
+                               "N2 type 6 alternating), %d trajectories each" % per,
+                   "targets": len(P), "atoms_total": int(sum(p.info.natoms for p in P)),
+                   "trajectories_per_step": n, "parallelism": "1 GPU"},
+        "force_evals_per_s": None,
+        "steps_per_trajectory": {"min": int(r["steps"][r["hit"] == 1].min()),
+                                 "max": int(r["steps"].max()),
+                                 "mean": float(r["steps"][r["hit"] == 1].mean())},
+        "autotune": {"seconds": round(t_tune, 2),
+                     "choices": [{"class": c["class"].split("/")[0] + "/" + c["class"].split("/")[1],
+                                  "lanes": c["lanes"], "skin": c["skin"]} for c in tune]},
+        "flops": _flops(r), "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,11 +236,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="globule",
+                    choices=["globule", "batch64", "ellipsoid50k", "shell1m"],
+                    help="globule: the measured configuration (1,231 atoms, N2, 10^5 "
+                         "trajectories); the others are the batch and large-target runs")
+    ap.add_argument("--traj", type=int, default=0, help="trajectories (per target for batch64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the trajectory engine is measured on a single "
               "MI355X (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch64":
+        print(json.dumps(run_batch(args)))
+        return
+    if args.workload in ("ellipsoid50k", "shell1m"):
+        print(json.dumps(run_big(args, args.workload)))
+        return
 
     import torch
     import massccs_amd as M
@@ -168,10 +303,7 @@ def main():
     executed = E.last_work()  # every try run, including side-by-side tries left unused
     r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
     me = mean_err.cpu().numpy()
-    hits = r["hit"] == 1
-    evals = (1 + r["steps"] + np.maximum(r["tries"], 0)) * hits
-    flops = (W_VISIT * r["visits"].sum() + W_LJ * r["lj"].sum() + W_COUL * r["coul"].sum()
-             + W_DIP * r["dip"].sum() + W_EPI * evals.sum())
+    flops = _flops(r)
     achieved = flops / (kern_ms * 1e-3) / 1e12
     value = n * args.steps / wall
     site_evals = 3 * int(r["steps"].sum())

@@ -201,6 +201,50 @@ mccs_status mccs_tally_device(mccs_engine *e, const mccs_traj_record *records_de
                               uint64_t n_iter, uint64_t n_traj, double *omega_dev,
                               int64_t *counts_dev, double *mean_err_dev, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Batches: many prepared targets in one run (no counterpart in the
+ * reference, which runs one target per process).  Targets are grouped by
+ * probe kind and force type; each group runs as one persistent launch whose
+ * device work queue spans all its targets' trajectories (larger targets
+ * first), and the launches are captured in a hipGraph that later runs of the
+ * same sizes replay.  Supported: He/Ar/co2 force types 2 and 4, N2 force
+ * type 6.  Records are those mccs_run would produce for each target with the
+ * same launch shape. */
+typedef struct mccs_batch mccs_batch;
+
+typedef struct {
+  int32_t device;
+  int32_t lanes;           /* 0: tuned (or 64 until mccs_batch_autotune runs), else 16/32/64 */
+  int32_t count_work;      /* 1: fill the work counters of the records */
+  int32_t use_graph;       /* 1: capture the launches in a hipGraph and replay it */
+  double skin;             /* Verlet-list skin in A; 0: tuned (or 1 A) */
+} mccs_batch_opts;
+
+/* The targets must outlive nothing: their data is copied to the device. */
+mccs_status mccs_batch_create(const mccs_prepared *const *targets, int32_t ntargets,
+                              const mccs_batch_opts *opts, mccs_batch **out);
+void mccs_batch_destroy(mccs_batch *b);
+
+/* Runs n[t] trajectories of every target t from host samples[t] (n[t] x 9
+ * doubles, as written by mccs_samples) into host records[t]. */
+mccs_status mccs_batch_run(mccs_batch *b, const uint64_t *n, const double *const *samples,
+                           mccs_traj_record *const *records);
+
+/* Autotuner: for every group, times trial runs of the first n_trial[t]
+ * samples of each target over lanes per trajectory {16, 32, 64} and Verlet
+ * skin {0.5, 1, 2} A, keeps the fastest, and caches it for this process by
+ * (probe, force type, cutoffs, target sizes).  report (may be NULL) receives
+ * the trials as JSON; len the full length. */
+mccs_status mccs_batch_autotune(mccs_batch *b, const uint64_t *n_trial,
+                                const double *const *samples, char *report, size_t cap,
+                                size_t *len);
+
+/* Launch shape chosen for every target (either array may be NULL). */
+mccs_status mccs_batch_shapes(const mccs_batch *b, int32_t *lanes, double *skin);
+
+/* Executed work of the last batch run, as mccs_engine_last_work. */
+mccs_status mccs_batch_last_work(mccs_batch *b, uint64_t totals[5]);
+
 /* Library version string. */
 const char *mccs_version(void);
 

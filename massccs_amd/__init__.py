@@ -59,6 +59,11 @@ class EngineOpts(C.Structure):
                 ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("count_work", C.c_int32),
+                ("use_graph", C.c_int32), ("skin", C.c_double)]
+
+
 RECORD_DTYPE = np.dtype([("hit", "<i4"), ("success", "<i4"), ("tries", "<i4"), ("steps", "<i4"),
                          ("chi", "<f8"), ("domega", "<f8"), ("visits", "<u8"), ("lj", "<u8"),
                          ("coul", "<u8"), ("dip", "<u8")])
@@ -69,7 +74,9 @@ EXPORTS = [
     "mccs_prepare_atoms", "mccs_prepared_destroy", "mccs_prepared_get_info",
     "mccs_prepared_get_atoms", "mccs_prepared_report", "mccs_samples", "mccs_engine_create",
     "mccs_engine_destroy", "mccs_run", "mccs_run_device", "mccs_eval_sites", "mccs_tally",
-    "mccs_tally_device", "mccs_version", "mccs_engine_last_work",
+    "mccs_tally_device", "mccs_version", "mccs_engine_last_work", "mccs_batch_create",
+    "mccs_batch_destroy", "mccs_batch_run", "mccs_batch_autotune", "mccs_batch_shapes",
+    "mccs_batch_last_work",
 ]
 
 _lib = None
@@ -108,13 +115,23 @@ def lib():
     L.mccs_eval_sites.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_engine_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.mccs_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(BatchOpts),
+                                    C.POINTER(C.c_void_p)]
+    L.mccs_batch_destroy.argtypes = [C.c_void_p]
+    L.mccs_batch_destroy.restype = None
+    L.mccs_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p)]
+    L.mccs_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_void_p),
+                                      C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.mccs_batch_shapes.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.mccs_batch_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.mccs_tally.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                              C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_tally_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]
     for n in EXPORTS:
         if n not in ("mccs_last_error", "mccs_version", "mccs_prepared_destroy",
-                     "mccs_engine_destroy"):
+                     "mccs_engine_destroy", "mccs_batch_destroy"):
             getattr(L, n).restype = st
     _lib = L
     return L
@@ -248,6 +265,64 @@ class Engine:
         U = np.zeros(len(xyz))
         _check(lib().mccs_eval_sites(self.h, site, _dp(xyz), len(xyz), _dp(f), _dp(U)))
         return f, U
+
+
+class Batch:
+    """Several prepared targets run together (one persistent launch per probe
+    kind / force type, replayed from a hipGraph)."""
+
+    def __init__(self, prepared_list, device=0, lanes=0, count_work=True, use_graph=True,
+                 skin=0.0):
+        self.targets = list(prepared_list)
+        arr = (C.c_void_p * len(self.targets))(*[p.h.value for p in self.targets])
+        o = BatchOpts(device, lanes, 1 if count_work else 0, 1 if use_graph else 0, skin)
+        h = C.c_void_p()
+        _check(lib().mccs_batch_create(arr, len(self.targets), C.byref(o), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mccs_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _ptrs(self, samples):
+        us = [np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 9) for u in samples]
+        assert len(us) == len(self.targets)
+        n = (C.c_uint64 * len(us))(*[len(u) for u in us])
+        up = (C.c_void_p * len(us))(*[u.ctypes.data for u in us])
+        return us, n, up
+
+    def run(self, samples):
+        """samples: one (n_t, 9) array per target; returns one record array each."""
+        us, n, up = self._ptrs(samples)
+        recs = [np.zeros(len(u), dtype=RECORD_DTYPE) for u in us]
+        rp = (C.c_void_p * len(us))(*[r.ctypes.data for r in recs])
+        _check(lib().mccs_batch_run(self.h, n, up, rp))
+        return recs
+
+    def autotune(self, samples):
+        """Trial runs over the given samples (a prefix of each target's);
+        returns the tuner's report (JSON)."""
+        import json
+        us, n, up = self._ptrs(samples)
+        ln = C.c_size_t()
+        buf = C.create_string_buffer(1 << 20)
+        _check(lib().mccs_batch_autotune(self.h, n, up, buf, len(buf), C.byref(ln)))
+        return json.loads(buf.value.decode())
+
+    def shapes(self):
+        k = len(self.targets)
+        lanes = (C.c_int32 * k)()
+        skin = (C.c_double * k)()
+        _check(lib().mccs_batch_shapes(self.h, lanes, skin))
+        return list(lanes), list(skin)
+
+    def last_work(self):
+        out = (C.c_uint64 * 5)()
+        _check(lib().mccs_batch_last_work(self.h, out))
+        return dict(zip(("visits", "lj", "coul", "dip", "steps"), list(out)))
 
 
 def tally(rec, n_iter, n_traj):

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "host.hpp"
@@ -12,6 +13,9 @@ struct mccs_prepared {
 };
 struct mccs_engine {
   mccs::Engine *e;
+};
+struct mccs_batch {
+  mccs::Batch *b;
 };
 
 namespace {
@@ -197,6 +201,68 @@ mccs_status mccs_engine_last_work(mccs_engine *e, uint64_t totals[5]) {
   return guard([&] {
     need(e && totals, "null argument");
     mccs::engine_last_work(e->e, totals);
+  });
+}
+
+mccs_status mccs_batch_create(const mccs_prepared *const *targets, int32_t ntargets,
+                              const mccs_batch_opts *opts, mccs_batch **out) {
+  return guard([&] {
+    need(targets && out && ntargets > 0, "null argument or empty batch");
+    *out = nullptr;
+    mccs_batch_opts o{};
+    o.use_graph = 1;
+    o.count_work = 1;
+    if (opts) o = *opts;
+    std::vector<const mccs::Prepared *> ps;
+    for (int32_t t = 0; t < ntargets; t++) {
+      need(targets[t] != nullptr, "null target");
+      ps.push_back(targets[t]->p);
+    }
+    auto *b = mccs::batch_create(ps.data(), ntargets, o);
+    *out = new mccs_batch{b};
+  });
+}
+
+void mccs_batch_destroy(mccs_batch *b) {
+  if (!b) return;
+  mccs::batch_destroy(b->b);
+  delete b;
+}
+
+mccs_status mccs_batch_run(mccs_batch *b, const uint64_t *n, const double *const *samples,
+                           mccs_traj_record *const *records) {
+  return guard([&] {
+    need(b && n && samples && records, "null argument");
+    mccs::batch_run(b->b, n, samples, records);
+  });
+}
+
+mccs_status mccs_batch_autotune(mccs_batch *b, const uint64_t *n_trial,
+                                const double *const *samples, char *report, size_t cap,
+                                size_t *len) {
+  return guard([&] {
+    need(b && n_trial && samples, "null argument");
+    const std::string r = mccs::batch_autotune(b->b, n_trial, samples);
+    if (len) *len = r.size();
+    if (report && cap) {
+      const size_t k = std::min(cap - 1, r.size());
+      std::memcpy(report, r.data(), k);
+      report[k] = 0;
+    }
+  });
+}
+
+mccs_status mccs_batch_shapes(const mccs_batch *b, int32_t *lanes, double *skin) {
+  return guard([&] {
+    need(b, "null argument");
+    mccs::batch_shapes(b->b, lanes, skin);
+  });
+}
+
+mccs_status mccs_batch_last_work(mccs_batch *b, uint64_t totals[5]) {
+  return guard([&] {
+    need(b && totals, "null argument");
+    mccs::batch_last_work(b->b, totals);
   });
 }
 

@@ -2,6 +2,11 @@
 // the host-side launcher behind the C ABI.
 #include "kernels.hpp"
 
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+
 namespace mccs {
 namespace dev {
 
@@ -114,6 +119,75 @@ constexpr int QLINE = 32;  // 64-bit words per counter line
 constexpr int QCTL_WORDS = 6 * QLINE;
 constexpr uint64_t SLOT_SLACK = 8192;  // spawn slots beyond 8 per trajectory
 
+// Device work queue of one launch: counters, each polled one on its own
+// 256-byte line (ticket, shead, stail, active, work[5], stalled), and a
+// per-trajectory workspace grown on demand.
+struct QueueWS {
+  unsigned long long *qctl = nullptr;
+  char *qws = nullptr;
+  uint64_t qcap = 0;
+  unsigned long long wait_ticks = 0;
+  int device = 0;
+
+  void init(int dev) {
+    device = dev;
+    HIP_OK(hipMalloc(&qctl, sizeof(unsigned long long) * QCTL_WORDS));
+    HIP_OK(hipMemset(qctl, 0, sizeof(unsigned long long) * QCTL_WORDS));
+    // a group waits for work at most 120 s (any single try takes far less)
+    int khz = 0;
+    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) khz = 100000;
+    wait_ticks = 120ull * 1000ull * (unsigned long long)khz;
+  }
+  // Queue over n trajectories; spawning below spec_below unresolved ones.
+  Queue make(uint64_t n, long long spec_below) {
+    // per trajectory: TRIES_MAX try outcomes, 8 spawn slots, claimed tries,
+    // spawned and resolved flags
+    constexpr size_t per_traj =
+        TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int);
+    if (n > qcap) {
+      if (qws) HIP_OK(hipFree(qws));
+      qws = nullptr;
+      qcap = 0;
+      const uint64_t cap = std::max<uint64_t>(n, 1024);
+      HIP_OK(hipMalloc(&qws, per_traj * cap + sizeof(unsigned long long) * SLOT_SLACK));
+      qcap = cap;
+    }
+    Queue Q;
+    Q.ticket = qctl + 0 * QLINE;
+    Q.shead = qctl + 1 * QLINE;
+    Q.stail = qctl + 2 * QLINE;
+    Q.active = reinterpret_cast<long long *>(qctl + 3 * QLINE);
+    Q.work = qctl + 4 * QLINE;
+    Q.stalled = reinterpret_cast<int *>(qctl + 5 * QLINE);
+    Q.tab = reinterpret_cast<TryOut *>(qws);
+    Q.slots = reinterpret_cast<unsigned long long *>(qws + TRIES_MAX * sizeof(TryOut) * n);
+    Q.nslots = 8 * n + SLOT_SLACK;
+    Q.tries = reinterpret_cast<int *>(Q.slots + Q.nslots);
+    Q.spawned = Q.tries + n;
+    Q.resolved = Q.spawned + n;
+    Q.wait_ticks = wait_ticks;
+    Q.spec_below = spec_below;
+    return Q;
+  }
+  // Throws when a group of the last run gave up waiting for work (its
+  // records are then incomplete).  The run must have completed.
+  void check_stall() const {
+    unsigned long long c = 0;
+    HIP_OK(hipMemcpy(&c, qctl + 5 * QLINE, sizeof c, hipMemcpyDeviceToHost));
+    if ((int)(c & 0xffffffffu) != 0)
+      throw Error(MCCS_ERR_DEVICE, "trajectory work queue stalled; records are incomplete");
+  }
+  void work(uint64_t out[5]) const {
+    HIP_OK(hipMemcpy(out, qctl + 4 * QLINE, sizeof(uint64_t) * 5, hipMemcpyDeviceToHost));
+  }
+  ~QueueWS() {
+    if (qctl || qws) hipSetDevice(device);
+    if (qctl) hipFree(qctl);
+    if (qws) hipFree(qws);
+  }
+};
+
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -123,13 +197,8 @@ struct Engine {
   Tgt T{};
   double site_q[5] = {0, -0.4825, 0.965, -0.3256, 0.6512};
   std::vector<void *> allocs;
-  // work queue: counters, each polled one on its own 256-byte line (ticket,
-  // shead, stail, active, work[5], stalled), and a per-trajectory workspace
-  // grown on demand
-  unsigned long long *qctl = nullptr;
-  char *qws = nullptr;
-  uint64_t qcap = 0;
-  unsigned long long wait_ticks = 0;
+  QueueWS q;
+  Tgt *d_tg = nullptr;  // T in device memory (the kernel's target array)
   int spec_below = -1;
   hipStream_t last_stream = nullptr;
   int nsites = 1;
@@ -239,14 +308,8 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
     T.rfar2 = rf * rf * (1.0 + 1e-12);
     T.rlong = longr ? rc : rlj;
   }
-  E->qctl = reinterpret_cast<unsigned long long *>(E->dalloc(sizeof(unsigned long long) * QCTL_WORDS));
-  HIP_OK(hipMemset(E->qctl, 0, sizeof(unsigned long long) * QCTL_WORDS));
-  {  // a group waits for work at most 120 s (any single try takes far less)
-    int khz = 0;
-    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E->device));
-    if (khz <= 0) khz = 100000;
-    E->wait_ticks = 120ull * 1000ull * (unsigned long long)khz;
-  }
+  T.skin = 1.0;
+  E->q.init(E->device);
 
   // launch shape: lanes per trajectory by target size unless given
   int L = o.lanes;
@@ -271,10 +334,19 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   T.list_cap = (int)std::min<size_t>((size_t)T.natoms, room / (lidx * groups));
   E->shmem = lds_bytes(T.natoms, T.list_cap, E->block, L, E->lds, central);
   if (E->shmem > 160 * 1024) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
+  E->d_tg = reinterpret_cast<Tgt *>(E->dalloc(sizeof(Tgt)));
+  HIP_OK(hipMemcpy(E->d_tg, &T, sizeof(Tgt), hipMemcpyHostToDevice));
   return E.release();
 }
 
 void engine_destroy(Engine *e) { delete e; }
+
+// Clears the queue; the launch's own status is checked (not the thread's
+// sticky last error, which unchecked teardown calls may have set).
+static void launch_queue_init(Queue Q, uint64_t n, hipStream_t s) {
+  void *args[] = {&Q, &n};
+  HIP_OK(hipLaunchKernel((const void *)queue_init, dim3(256), dim3(256), args, 0, s));
+}
 
 static void set_shmem(const void *fn, size_t bytes) {
   if (bytes > 64 * 1024)
@@ -300,60 +372,23 @@ void engine_run_device(Engine *E, const double *u_dev, uint64_t n, mccs_traj_rec
   const uint64_t groups_per_block = E->block / E->lanes;
   uint64_t blocks = (uint64_t)per_cu * E->cus;
   blocks = std::min<uint64_t>(blocks, (n + groups_per_block - 1) / groups_per_block);
-  // per trajectory: TRIES_MAX try outcomes, 8 spawn slots, claimed tries,
-  // spawned and resolved flags
-  constexpr size_t per_traj = TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int);
-  if (n > E->qcap) {
-    if (E->qws) {
-      auto it = std::find(E->allocs.begin(), E->allocs.end(), (void *)E->qws);
-      if (it != E->allocs.end()) E->allocs.erase(it);
-      HIP_OK(hipFree(E->qws));
-      E->qws = nullptr;
-      E->qcap = 0;
-    }
-    const uint64_t cap = std::max<uint64_t>(n, 1024);
-    E->qws = reinterpret_cast<char *>(E->dalloc(per_traj * cap + sizeof(unsigned long long) * SLOT_SLACK));
-    E->qcap = cap;
-  }
-  Queue Q;
-  Q.ticket = E->qctl + 0 * QLINE;
-  Q.shead = E->qctl + 1 * QLINE;
-  Q.stail = E->qctl + 2 * QLINE;
-  Q.active = reinterpret_cast<long long *>(E->qctl + 3 * QLINE);
-  Q.work = E->qctl + 4 * QLINE;
-  Q.stalled = reinterpret_cast<int *>(E->qctl + 5 * QLINE);
-  Q.tab = reinterpret_cast<TryOut *>(E->qws);
-  Q.slots = reinterpret_cast<unsigned long long *>(E->qws + TRIES_MAX * sizeof(TryOut) * n);
-  Q.nslots = 8 * n + SLOT_SLACK;
-  Q.tries = reinterpret_cast<int *>(Q.slots + Q.nslots);
-  Q.spawned = Q.tries + n;
-  Q.resolved = Q.spawned + n;
-  Q.wait_ticks = E->wait_ticks;
   // automatic: spawning starts once fewer trajectories than resident groups
   // are unresolved, i.e. when groups begin to fall idle
-  Q.spec_below = E->spec_below >= 0 ? E->spec_below : (long long)(blocks * groups_per_block);
-  hipLaunchKernelGGL(queue_init, dim3(256), dim3(256), 0, s, Q, n);
-  HIP_OK(hipGetLastError());
-  Tgt T = E->T;
-  void *args[] = {&T, &u_dev, &n, &rec_dev, &Q};
+  Queue Q = E->q.make(n, E->spec_below >= 0 ? E->spec_below : (long long)(blocks * groups_per_block));
+  launch_queue_init(Q, n, s);
+  const Tgt *tg = E->d_tg;
+  const unsigned long long *toff = nullptr;
+  int ntg = 1;
+  void *args[] = {&tg, &toff, &ntg, &u_dev, &n, &rec_dev, &Q};
   HIP_OK(hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(E->block), args, E->shmem, s));
   E->last_stream = s;
-}
-
-// Throws when a group of the last run gave up waiting for work (its records
-// are then incomplete).
-static void check_stall(Engine *E) {
-  unsigned long long c = 0;
-  HIP_OK(hipMemcpy(&c, E->qctl + 5 * QLINE, sizeof c, hipMemcpyDeviceToHost));
-  if ((int)(c & 0xffffffffu) != 0)
-    throw Error(MCCS_ERR_DEVICE, "trajectory work queue stalled; records are incomplete");
 }
 
 void engine_last_work(Engine *E, uint64_t out[5]) {
   HIP_OK(hipSetDevice(E->device));
   HIP_OK(hipStreamSynchronize(E->last_stream ? E->last_stream : E->stream));
-  check_stall(E);
-  HIP_OK(hipMemcpy(out, E->qctl + 4 * QLINE, sizeof(uint64_t) * 5, hipMemcpyDeviceToHost));
+  E->q.check_stall();
+  E->q.work(out);
 }
 
 void engine_run_host(Engine *E, const double *u, uint64_t n, mccs_traj_record *rec) {
@@ -371,7 +406,7 @@ void engine_run_host(Engine *E, const double *u, uint64_t n, mccs_traj_record *r
     engine_run_device(E, du, n, dr, E->stream);
     HIP_OK(hipMemcpyAsync(rec, dr, sizeof(mccs_traj_record) * n, hipMemcpyDeviceToHost, E->stream));
     HIP_OK(hipStreamSynchronize(E->stream));
-    check_stall(E);
+    E->q.check_stall();
   } catch (...) {
     hipFree(du);
     hipFree(dr);
@@ -446,6 +481,373 @@ void engine_tally_device(Engine *E, const mccs_traj_record *rec_dev, uint64_t n_
                      omega_dev, counts_dev);
   hipLaunchKernelGGL(tally_final_kernel, dim3(1), dim3(64), 0, s, omega_dev, n_iter, mean_err_dev);
   HIP_OK(hipGetLastError());
+}
+
+
+// ---------------------------------------------------------------- batches
+// Many targets in one run: targets are grouped by probe kind and force type
+// (one kernel instantiation each); each group runs as ONE persistent launch
+// whose device queue spans all its targets' trajectories, ordered by
+// predicted cost (largest targets first) so that the long ones start early.
+// The launches are captured once in a hipGraph and replayed.
+namespace {
+
+struct BatchClass {
+  int gk = 1, ft = 2;
+  std::vector<int> members;  // target indices, in queue order
+  int lanes = 64, block = 256;
+  double skin = 1.0;
+  bool tuned = false;
+  Tgt *d_tg = nullptr;
+  unsigned long long *d_toff = nullptr;
+  size_t shmem = 0;
+  uint64_t blocks = 0, n = 0;
+  TrajFn fn = nullptr;
+  QueueWS q;
+  double *d_u = nullptr;
+  mccs_traj_record *d_rec = nullptr;
+  uint64_t cap = 0;
+  std::vector<unsigned long long> toff;
+};
+
+// Tuned launch shapes by class signature: probe kind, force type, cutoffs and
+// the targets' sizes rounded to powers of two.
+std::map<std::string, std::pair<int, double>> &tune_cache() {
+  static std::map<std::string, std::pair<int, double>> c;
+  return c;
+}
+
+}  // namespace
+
+struct Batch {
+  int device = 0, cus = 0;
+  bool count = true, use_graph = true;
+  int lanes_opt = 0;
+  double skin_opt = 0.0;
+  hipStream_t stream = nullptr;
+  std::vector<Engine *> eng;  // per target: device-resident target data
+  std::vector<std::string> sig;
+  std::vector<BatchClass> cls;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gkey;
+  ~Batch() {
+    hipSetDevice(device);
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    for (auto &c : cls) {
+      if (c.d_tg) hipFree(c.d_tg);
+      if (c.d_toff) hipFree(c.d_toff);
+      if (c.d_u) hipFree(c.d_u);
+      if (c.d_rec) hipFree(c.d_rec);
+    }
+    for (Engine *e : eng) delete e;
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+static std::string class_signature(const Batch &B, const BatchClass &c) {
+  std::ostringstream o;
+  o << c.gk << "/" << c.ft;
+  for (int t : c.members) {
+    const Tgt &T = B.eng[t]->T;
+    int lg = 0;
+    while ((1 << lg) < T.natoms) lg++;
+    o << "/" << lg << ":" << T.rlj << ":" << T.rc;
+  }
+  return o.str();
+}
+
+// (Re)builds a class's device target array for its launch shape: list
+// capacity for the shape's LDS budget and the chosen skin.
+static void configure_class(Batch &B, BatchClass &c) {
+  c.fn = c.gk == 1 ? select_traj_multi1(c.ft, c.lanes, B.count)
+                   : (c.gk == 2 ? select_traj_multi2(c.ft, c.lanes, B.count) : nullptr);
+  if (!c.fn)
+    throw Error(MCCS_ERR_UNSUPPORTED,
+                "batched runs support He/Ar/co2 force types 2 and 4 and N2 force type 6");
+  const int groups = c.block / c.lanes;
+  const bool central = c.gk == 3;
+  const size_t base = lds_bytes(0, 0, c.block, c.lanes, false, central);
+  const size_t room = base < 150 * 1024 ? 150 * 1024 - base : 0;
+  const int cap = (int)(room / (sizeof(int) * groups));
+  std::vector<Tgt> tg;
+  int maxcap = 0;
+  for (int t : c.members) {
+    Tgt T = B.eng[t]->T;
+    T.list_cap = std::min(T.natoms, cap);
+    T.skin = c.skin;
+    maxcap = std::max(maxcap, T.list_cap);
+    tg.push_back(T);
+  }
+  c.shmem = lds_bytes(0, maxcap, c.block, c.lanes, false, central);
+  if (!c.d_tg) HIP_OK(hipMalloc(&c.d_tg, sizeof(Tgt) * tg.size()));
+  HIP_OK(hipMemcpy(c.d_tg, tg.data(), sizeof(Tgt) * tg.size(), hipMemcpyHostToDevice));
+  set_shmem((const void *)c.fn, c.shmem);
+  int per_cu = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)c.fn, c.block, c.shmem));
+  if (per_cu < 1) throw Error(MCCS_ERR_DEVICE, "batched trajectory kernel does not fit on a CU");
+  c.blocks = (uint64_t)per_cu * B.cus;
+}
+
+Batch *batch_create(const Prepared *const *targets, int ntargets, const mccs_batch_opts &o) {
+  if (ntargets <= 0) throw Error(MCCS_ERR_INVALID, "a batch needs at least one target");
+  std::unique_ptr<Batch> B(new Batch);
+  B->device = o.device;
+  B->count = o.count_work != 0;
+  B->use_graph = o.use_graph != 0;
+  B->lanes_opt = o.lanes;
+  B->skin_opt = o.skin;
+  if (o.lanes != 0 && o.lanes != 16 && o.lanes != 32 && o.lanes != 64)
+    throw Error(MCCS_ERR_INVALID, "lanes must be 0, 16, 32 or 64");
+  if (o.skin < 0.0) throw Error(MCCS_ERR_INVALID, "skin must be >= 0 (0 = tuned)");
+  mccs_engine_opts eo{};
+  eo.device = o.device;
+  eo.lanes = 64;
+  eo.use_lds = 0;
+  eo.count_work = o.count_work;
+  eo.spec_below = -1;
+  for (int t = 0; t < ntargets; t++) {
+    if (!targets[t]) throw Error(MCCS_ERR_INVALID, "null target");
+    B->eng.push_back(engine_create(*targets[t], eo));
+  }
+  B->device = B->eng[0]->device;
+  B->cus = B->eng[0]->cus;
+  HIP_OK(hipSetDevice(B->device));
+  HIP_OK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+  // classes by (probe kind, force type); members by predicted cost: atoms
+  // within the longest cutoff grow with the target, so larger targets first
+  std::map<std::pair<int, int>, std::vector<int>> by;
+  for (int t = 0; t < ntargets; t++) by[{B->eng[t]->gk, B->eng[t]->ft}].push_back(t);
+  for (auto &kv : by) {
+    B->cls.emplace_back();
+    BatchClass &c = B->cls.back();
+    c.gk = kv.first.first;
+    c.ft = kv.first.second;
+    c.members = kv.second;
+    std::stable_sort(c.members.begin(), c.members.end(),
+                     [&](int a, int b) { return B->eng[a]->T.natoms > B->eng[b]->T.natoms; });
+    c.q.init(B->device);
+    c.lanes = o.lanes ? o.lanes : 64;
+    c.skin = o.skin > 0.0 ? o.skin : 1.0;
+    auto it = tune_cache().find(class_signature(*B, c));
+    if (it != tune_cache().end() && !o.lanes && !(o.skin > 0.0)) {
+      c.lanes = it->second.first;
+      c.skin = it->second.second;
+      c.tuned = true;
+    }
+    configure_class(*B, c);
+  }
+  return B.release();
+}
+
+void batch_destroy(Batch *b) { delete b; }
+
+// Uploads the samples of n[t] trajectories per target into each class's
+// concatenated buffers; returns whether any class changed size.
+static bool stage_samples(Batch &B, const uint64_t *n, const double *const *samples) {
+  bool changed = false;
+  for (auto &c : B.cls) {
+    std::vector<unsigned long long> off(1, 0);
+    for (int t : c.members) off.push_back(off.back() + n[t]);
+    const uint64_t tot = off.back();
+    if (tot > c.cap) {
+      if (c.d_u) HIP_OK(hipFree(c.d_u));
+      if (c.d_rec) HIP_OK(hipFree(c.d_rec));
+      c.d_u = nullptr;
+      c.d_rec = nullptr;
+      HIP_OK(hipMalloc(&c.d_u, sizeof(double) * 9 * std::max<uint64_t>(tot, 1)));
+      HIP_OK(hipMalloc(&c.d_rec, sizeof(mccs_traj_record) * std::max<uint64_t>(tot, 1)));
+      c.cap = tot;
+      changed = true;
+    }
+    if (off != c.toff) changed = true;
+    c.toff = off;
+    c.n = tot;
+    if (!c.d_toff) HIP_OK(hipMalloc(&c.d_toff, sizeof(unsigned long long) * (c.members.size() + 1)));
+    HIP_OK(hipMemcpyAsync(c.d_toff, off.data(), sizeof(unsigned long long) * off.size(),
+                          hipMemcpyHostToDevice, B.stream));
+    for (size_t m = 0; m < c.members.size(); m++) {
+      const int t = c.members[m];
+      if (n[t] && !samples[t]) throw Error(MCCS_ERR_INVALID, "null samples");
+      if (n[t])
+        HIP_OK(hipMemcpyAsync(c.d_u + 9 * off[m], samples[t], sizeof(double) * 9 * n[t],
+                              hipMemcpyHostToDevice, B.stream));
+    }
+  }
+  return changed;
+}
+
+static void launch_class(Batch &B, BatchClass &c, hipStream_t s) {
+  if (c.n == 0) return;
+  const uint64_t gpb = c.block / c.lanes;
+  const uint64_t blocks = std::min<uint64_t>(c.blocks, (c.n + gpb - 1) / gpb);
+  Queue Q = c.q.make(c.n, (long long)(blocks * gpb));
+  launch_queue_init(Q, c.n, s);
+  const Tgt *tg = c.d_tg;
+  const unsigned long long *toff = c.d_toff;
+  int ntg = (int)c.members.size();
+  const double *u = c.d_u;
+  uint64_t n = c.n;
+  mccs_traj_record *rec = c.d_rec;
+  void *args[] = {&tg, &toff, &ntg, &u, &n, &rec, &Q};
+  HIP_OK(hipLaunchKernel((const void *)c.fn, dim3((unsigned)blocks), dim3(c.block), args, c.shmem, s));
+}
+
+// Runs every class on the batch stream: replays the captured graph when the
+// sizes and shapes are those it was captured with.
+static void run_classes(Batch &B, bool sizes_changed) {
+  if (!B.use_graph) {
+    for (auto &c : B.cls) launch_class(B, c, B.stream);
+    return;
+  }
+  std::vector<uint64_t> key;
+  for (auto &c : B.cls) {
+    key.push_back(c.n);
+    key.push_back((uint64_t)c.lanes);
+    key.push_back((uint64_t)(c.skin * 1000.0));
+  }
+  if (!B.gexec || sizes_changed || key != B.gkey) {
+    if (B.gexec) HIP_OK(hipGraphExecDestroy(B.gexec));
+    if (B.graph) HIP_OK(hipGraphDestroy(B.graph));
+    B.gexec = nullptr;
+    B.graph = nullptr;
+    // every queue workspace must exist before capture (no allocation inside)
+    for (auto &c : B.cls) c.q.make(c.n, 0);
+    HIP_OK(hipStreamSynchronize(B.stream));
+    HIP_OK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (auto &c : B.cls) launch_class(B, c, B.stream);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      hipStreamEndCapture(B.stream, &g);
+      if (g) hipGraphDestroy(g);
+      throw;
+    }
+    HIP_OK(hipStreamEndCapture(B.stream, &B.graph));
+    HIP_OK(hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0));
+    B.gkey = key;
+  }
+  HIP_OK(hipGraphLaunch(B.gexec, B.stream));
+}
+
+static void fetch_records(Batch &B, const uint64_t *n, mccs_traj_record *const *records) {
+  for (auto &c : B.cls)
+    for (size_t m = 0; m < c.members.size(); m++) {
+      const int t = c.members[m];
+      if (n[t] && !records[t]) throw Error(MCCS_ERR_INVALID, "null records");
+      if (n[t])
+        HIP_OK(hipMemcpyAsync(records[t], c.d_rec + c.toff[m], sizeof(mccs_traj_record) * n[t],
+                              hipMemcpyDeviceToHost, B.stream));
+    }
+  HIP_OK(hipStreamSynchronize(B.stream));
+  for (auto &c : B.cls) c.q.check_stall();
+}
+
+void batch_run(Batch *B, const uint64_t *n, const double *const *samples,
+               mccs_traj_record *const *records) {
+  HIP_OK(hipSetDevice(B->device));
+  const bool changed = stage_samples(*B, n, samples);
+  run_classes(*B, changed);
+  fetch_records(*B, n, records);
+}
+
+void batch_last_work(Batch *B, uint64_t out[5]) {
+  HIP_OK(hipSetDevice(B->device));
+  HIP_OK(hipStreamSynchronize(B->stream));
+  for (int k = 0; k < 5; k++) out[k] = 0;
+  for (auto &c : B->cls) {
+    uint64_t w[5];
+    c.q.work(w);
+    for (int k = 0; k < 5; k++) out[k] += w[k];
+  }
+}
+
+// Autotuner: per class, times trial runs (the first n_trial[t] samples of
+// every target) over lanes {16, 32, 64} x Verlet skin {0.5, 1, 2} A and
+// keeps the fastest; results are cached for this process by class signature.
+std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *const *samples) {
+  HIP_OK(hipSetDevice(B->device));
+  std::ostringstream rep;
+  rep << "[";
+  const int lanes_c[3] = {16, 32, 64};
+  const double skin_c[3] = {0.5, 1.0, 2.0};
+  std::vector<uint64_t> nt(B->eng.size());
+  for (size_t t = 0; t < nt.size(); t++) nt[t] = n_trial[t];
+  bool first = true;
+  for (auto &c : B->cls) {
+    std::vector<std::pair<int, double>> cand;
+    for (int L : lanes_c)
+      for (double sk : skin_c) {
+        if (B->lanes_opt && L != B->lanes_opt) continue;
+        if (B->skin_opt > 0.0 && sk != B->skin_opt) continue;
+        cand.push_back({L, sk});
+      }
+    if (B->skin_opt > 0.0 && cand.empty()) cand.push_back({B->lanes_opt ? B->lanes_opt : 64, B->skin_opt});
+    // trial samples of this class's members only
+    std::vector<uint64_t> n_only(B->eng.size(), 0);
+    for (int t : c.members) n_only[t] = nt[t];
+    double best = 1e300;
+    std::pair<int, double> pick{c.lanes, c.skin};
+    if (!first) rep << ",";
+    first = false;
+    rep << "{\"class\": \"" << class_signature(*B, c) << "\", \"trials\": [";
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    bool f2 = true;
+    for (auto &cd : cand) {
+      c.lanes = cd.first;
+      c.skin = cd.second;
+      float ms = 0.f;
+      try {
+        configure_class(*B, c);
+        stage_samples(*B, n_only.data(), samples);
+        HIP_OK(hipEventRecord(e0, B->stream));
+        launch_class(*B, c, B->stream);
+        HIP_OK(hipEventRecord(e1, B->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        c.q.check_stall();
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      } catch (const Error &e) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        throw Error(e.code, "autotune trial (lanes " + std::to_string(cd.first) + ", skin " +
+                                std::to_string(cd.second) + "): " + e.what());
+      }
+      if (!f2) rep << ", ";
+      f2 = false;
+      rep << "{\"lanes\": " << cd.first << ", \"skin\": " << cd.second << ", \"ms\": " << ms << "}";
+      if (ms < best) {
+        best = ms;
+        pick = cd;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c.lanes = pick.first;
+    c.skin = pick.second;
+    c.tuned = true;
+    configure_class(*B, c);
+    tune_cache()[class_signature(*B, c)] = pick;
+    rep << "], \"lanes\": " << pick.first << ", \"skin\": " << pick.second << "}";
+  }
+  rep << "]";
+  // the graph (if any) was captured with the old shapes
+  if (B->gexec) HIP_OK(hipGraphExecDestroy(B->gexec));
+  if (B->graph) HIP_OK(hipGraphDestroy(B->graph));
+  B->gexec = nullptr;
+  B->graph = nullptr;
+  B->gkey.clear();
+  return rep.str();
+}
+
+void batch_shapes(const Batch *B, int32_t *lanes, double *skin) {
+  for (const auto &c : B->cls)
+    for (int t : c.members) {
+      if (lanes) lanes[t] = c.lanes;
+      if (skin) skin[t] = c.skin;
+    }
 }
 
 }  // namespace mccs

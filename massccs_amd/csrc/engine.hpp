@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 
 #include "host.hpp"
 
@@ -26,5 +27,16 @@ void engine_last_work(Engine *e, uint64_t out[5]);
 void engine_tally_device(Engine *e, const mccs_traj_record *rec_dev, uint64_t n_iter,
                          uint64_t n_traj, double *omega_dev, int64_t *counts_dev,
                          double *mean_err_dev, void *stream);
+
+// Several targets in one run (one persistent launch per probe kind / force
+// type, captured in a hipGraph).
+struct Batch;
+Batch *batch_create(const Prepared *const *targets, int ntargets, const mccs_batch_opts &opts);
+void batch_destroy(Batch *b);
+void batch_run(Batch *b, const uint64_t *n, const double *const *samples,
+               mccs_traj_record *const *records);
+std::string batch_autotune(Batch *b, const uint64_t *n_trial, const double *const *samples);
+void batch_last_work(Batch *b, uint64_t out[5]);
+void batch_shapes(const Batch *b, int32_t *lanes, double *skin);
 
 }  // namespace mccs

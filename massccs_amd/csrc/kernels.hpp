@@ -50,7 +50,6 @@ constexpr double KC = 332.06348078020324;                    // KCOUL
 constexpr double PI = 3.14159265358979323846;
 constexpr double FAR = 1.0e30;                               // parks an inactive site
 constexpr int MAX_STEPS = 4000000;                           // per try
-constexpr double SKIN = 1.0;                                 // Verlet-list skin (A)
 
 enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4 };
 
@@ -75,6 +74,7 @@ struct Tgt {
   double blo[3], bhi[3];    // bounding box of the target atoms
   double rfar2;             // no term reaches beyond this squared distance
   double rlong;             // longest cutoff of the run's terms
+  double skin;              // Verlet-list skin (A)
 };
 
 // ---------------------------------------------------------------- probes
@@ -597,7 +597,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       const double moved = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
       if (vl->valid < 0 || moved + off + T.rlong + 1e-9 > vl->R) {
         VList nl;
-        build_list<S, L, LDS>(T, sh, lane, nl, c, T.rlong + off + SKIN);
+        build_list<S, L, LDS>(T, sh, lane, nl, c, T.rlong + off + T.skin);
         if (lane == 0) *vl = nl;
         lds_fence();
       }
@@ -877,7 +877,8 @@ struct GState {
   double Ei, dt, dtheta, hdm, Utot;
   Rat rc;
   VList vl;
-  unsigned long long j;         // trajectory
+  unsigned long long j;         // trajectory (index over all targets of the launch)
+  int t;                        // its target
   int k;                        // try (restart) index of the trajectory, 0..TRIES_MAX-1
   unsigned long long slot;      // spawn slot taken while its item is not yet written
   int steps;                    // steps of this try
@@ -1068,21 +1069,37 @@ __device__ bool resolve(const Queue &Q, mccs_traj_record *rec, uint64_t j, doubl
 // (trajectory, try) until every trajectory is resolved; per loop iteration
 // every working group performs exactly one force evaluation: the initial
 // force of a try, or the force of one velocity-Verlet/RATTLE step.
-template <int GK, int FT, int L, bool LDS, bool COUNT>
+template <int GK, int FT, int L, bool LDS, bool COUNT, bool MT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void traj_kernel(
-    Tgt T, const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec,
-    Queue Q) {
+    const Tgt *__restrict__ tg, const unsigned long long *__restrict__ toff, int ntg,
+    const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec, Queue Q) {
+  static_assert(!(MT && LDS), "targets staged in LDS need a launch of their own");
   using S = Spec<GK, FT>;
   constexpr bool CEN = any_central<S>();
   extern __shared__ __align__(16) char smem[];
   Shm sh;
-  stage<L, LDS, CEN>(T, sh, smem);
+  stage<L, LDS, CEN>(tg[0], sh, smem);
   const int lane = threadIdx.x % L;
   GState &st = *sh.state;
   constexpr int PA = 0, PB = GK == 2 ? 1 : 2;  // the constrained pair (N-N or O-O)
   constexpr double MA = S::NS == 1 ? 1.0 : S::m(PA), MB = S::NS == 1 ? 1.0 : S::m(PB);
-  const double iea = 1.0 / T.ea, ieb = 1.0 / T.eb, iec = 1.0 / T.ec;
-
+  // target of trajectory j: the last t with toff[t] <= j (group-uniform)
+  auto target_of = [&](uint64_t j) -> int {
+    if (!MT) return 0;
+    int lo = 0, hi = ntg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (toff[mid] <= j) lo = mid;
+      else hi = mid - 1;
+    }
+    return L == 64 ? __builtin_amdgcn_readfirstlane(lo) : lo;
+  };
+  // the group's current target; for a whole-wave group the index is made
+  // wave-uniform so that target fields are read with scalar loads
+  auto cur = [&]() -> int {
+    if (!MT) return 0;
+    return L == 64 ? __builtin_amdgcn_readfirstlane(st.t) : st.t;
+  };
   double qs[S::NS];
 #pragma unroll
   for (int s = 0; s < S::NS; s++) qs[s] = S::q(s);
@@ -1094,6 +1111,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     st.vl.valid = -1;
     st.cont = 0;
     st.has_slot = 0;
+    st.t = 0;
   }
   lds_fence();
 
@@ -1102,12 +1120,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     if (status == ST_NEED) {
       double pos[3][3], vel[3][3], rcm[3], vcm[3];
       uint64_t j = 0;
-      int k = 0;
+      int k = 0, tj = 0;
       bool got = false, wait = false, done = false;
       if (st.cont) {  // the next try of the trajectory whose try just failed
         j = st.j;
         k = st.k;
-        setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
+        tj = cur();
+        setup<S>(tg[tj], u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
         got = true;
       }
       while (!got && !wait && !done) {
@@ -1121,11 +1140,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         if (t < n) {
           j = t;
           k = 0;
+          tj = target_of(j);
           w = Work{0, 0, 0, 0};
-          if (setup<S>(T, u + 9 * j, pos, vel, rcm, vcm)) {
+          if (setup<S>(tg[tj], u + 9 * j, pos, vel, rcm, vcm)) {
             got = true;
           } else {
-            write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, T.bmax, w, COUNT);
+            write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, tg[tj].bmax, w, COUNT);
             if (lane == 0) {
               __threadfence();
               atomicAdd((unsigned long long *)Q.active, ~0ull);
@@ -1162,7 +1182,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
             if (moot) continue;  // the trajectory is resolved: take the next one
             j = item >> 4;
             k = (int)(item & 15);
-            setup<S>(T, u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
+            tj = target_of(j);
+            setup<S>(tg[tj], u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
             got = true;
             break;
           }
@@ -1214,11 +1235,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
           st.vcm[d] = vcm[d];
         }
         st.j = j;
+        st.t = tj;
         st.k = k;
         st.cont = 0;
       }
       status = ST_INIT;
     } else {  // ST_STEP: first half
+      const Tgt &T = tg[cur()];
       const double dt = st.dt;
       double rcm[3];
       if (S::NS == 1) {
@@ -1271,6 +1294,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
       }
     }
     lds_fence();
+    const Tgt &T = tg[cur()];
 
     // ---- phase B: one force evaluation for every working group
     {
@@ -1359,8 +1383,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     } else if (st.dtheta > 1.5 * PI) {
       code = TRY_ANGLE;
     } else if (fabs(rcm[0]) >= T.hx || fabs(rcm[1]) >= T.hy || fabs(rcm[2]) >= T.hz ||
-               (rcm[0] * iea) * (rcm[0] * iea) + (rcm[1] * ieb) * (rcm[1] * ieb) +
-                       (rcm[2] * iec) * (rcm[2] * iec) >
+               (rcm[0] / T.ea) * (rcm[0] / T.ea) + (rcm[1] / T.eb) * (rcm[1] / T.eb) +
+                       (rcm[2] / T.ec) * (rcm[2] / T.ec) >
                    1.0) {
       const double Ei = st.Ei, Ut = st.Utot;
       const double dE = S::NS == 1 ? fabs(Ut + Ek - Ei) / Ei * 100.0
@@ -1481,12 +1505,14 @@ static __global__ void queue_init(Queue Q, uint64_t n) {
   }
 }
 
-using TrajFn = void (*)(Tgt, const double *, uint64_t, mccs_traj_record *, Queue);
+using TrajFn = void (*)(const Tgt *, const unsigned long long *, int, const double *, uint64_t,
+                        mccs_traj_record *, Queue);
 
 template <int GK, int FT, int L>
 TrajFn pick_traj(bool lds, bool count) {
-  if (lds) return count ? traj_kernel<GK, FT, L, true, true> : traj_kernel<GK, FT, L, true, false>;
-  return count ? traj_kernel<GK, FT, L, false, true> : traj_kernel<GK, FT, L, false, false>;
+  if (lds)
+    return count ? traj_kernel<GK, FT, L, true, true, false> : traj_kernel<GK, FT, L, true, false, false>;
+  return count ? traj_kernel<GK, FT, L, false, true, false> : traj_kernel<GK, FT, L, false, false, false>;
 }
 
 // Tuned force types get every launch shape; the others only 64 lanes with
@@ -1503,12 +1529,24 @@ TrajFn pick_traj_full(int L, bool lds, bool count) {
 template <int GK, int FT>
 TrajFn pick_traj_basic(int L, bool lds, bool count) {
   if (L != 64 || !count) return nullptr;
-  return lds ? traj_kernel<GK, FT, 64, true, true> : traj_kernel<GK, FT, 64, false, true>;
+  return lds ? traj_kernel<GK, FT, 64, true, true, false> : traj_kernel<GK, FT, 64, false, true, false>;
+}
+
+// Launches over several targets (atoms read from HBM/L2, never staged).
+template <int GK, int FT>
+TrajFn pick_traj_multi(int L, bool count) {
+  switch (L) {
+    case 16: return count ? traj_kernel<GK, FT, 16, false, true, true> : traj_kernel<GK, FT, 16, false, false, true>;
+    case 32: return count ? traj_kernel<GK, FT, 32, false, true, true> : traj_kernel<GK, FT, 32, false, false, true>;
+    default: return count ? traj_kernel<GK, FT, 64, false, true, true> : traj_kernel<GK, FT, 64, false, false, true>;
+  }
 }
 
 TrajFn select_traj_gk1(int ft, int L, bool lds, bool count);
 TrajFn select_traj_gk2(int ft, int L, bool lds, bool count);
 TrajFn select_traj_gk3(int ft, int L, bool lds, bool count);
+TrajFn select_traj_multi1(int ft, int L, bool count);
+TrajFn select_traj_multi2(int ft, int L, bool count);
 
 }  // namespace dev
 }  // namespace mccs

@@ -7,6 +7,10 @@
              50,000-atom ellipsoid; above 20,000 atoms candidates are added
              in batches)
 * shell():   the same inside a hollow sphere (1,000,000-atom case)
+* assemblies(): the 64-target batch: atom counts log-uniform in
+             [500, 200000], ellipsoids with axis ratios U[1, 2] at the
+             1,231-atom globule's number density, net charge round(sqrt(N)/8),
+             probes alternating He / N2
 
 All write the reference's .xyz format (src/MoleculeTarget.cpp:148-319):
 count line, comment line, then "Symbol x y z [q]".
@@ -201,3 +205,20 @@ def write_xyz(path, symbols, xyz, q=None, comment="synthetic"):
                 fh.write("%s %.6f %.6f %.6f\n" % (s, r[0], r[1], r[2]))
             else:
                 fh.write("%s %.6f %.6f %.6f %.6f\n" % (s, r[0], r[1], r[2], q[i]))
+
+
+GLOBULE_DENSITY = 1231 / (4.0 / 3.0 * math.pi * 18.0 * 15.0 * 13.0)  # atoms per A^3
+
+
+def assemblies(count=64, n_min=500, n_max=200_000, seed=2104):
+    """Descriptions of the batch targets: list of dicts with n, axes, net
+    charge, probe and a per-target seed (atoms come from globule())."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        ra, rb = rng.uniform(1.0, 2.0, size=2)
+        s = (n / GLOBULE_DENSITY / (4.0 / 3.0 * math.pi * ra * rb)) ** (1.0 / 3.0)
+        out.append({"n": n, "axes": (s * ra, s * rb, s), "net_charge": float(max(1, round(math.sqrt(n) / 8))),
+                    "probe": "He" if k % 2 == 0 else "N2", "seed": seed + 1 + k})
+    return out
