@@ -149,6 +149,31 @@ def run_big(args, which):
     om, cnt, mean, err = M.tally(r, 1, n)
     flops = _flops(r)
     kern_ms = e0.elapsed_time(e1)
+    sur = None
+    if args.surrogate:
+        t0 = time.perf_counter()
+        rep, _, _ = E.build_surrogate()
+        t_build = time.perf_counter() - t0
+        E.use_surrogate(True)
+        E.run_device(u.data_ptr(), min(n, 20000), rec.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            E.run_device(u.data_ptr(), n, rec.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        wall_s = time.perf_counter() - t0
+        E.use_surrogate(False)
+        rs = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+        _, _, mean_s, _ = M.tally(rs, 1, n)
+        sur = {"value": round(n * args.steps / wall_s, 1), "unit": "trajectories/s",
+               "speedup_vs_plain": round(wall / wall_s, 3),
+               "ccs": mean_s, "ccs_error_vs_plain": round(mean_s / mean - 1, 5),
+               "lost": int(((rs["hit"] == 1) & (rs["success"] == 0)).sum()),
+               "build_seconds": round(t_build, 2),
+               "visits_ratio": round(float(rs["visits"].sum()) / max(1.0, float(r["visits"].sum())), 4),
+               "validation": {"u_rel": [rep["val_rmse_u"][k] / rep["val_rms_u"][k] for k in range(2)],
+                              "f_rel": [rep["val_rmse_f"][k] / rep["val_rms_f"][k] for k in range(2)]},
+               "grid_nodes": rep["grid_nodes"]}
     return {
         "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "trajectories/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -166,6 +191,7 @@ def run_big(args, which):
                      "traffic": None, "kernel": "traj_kernel<N2, type 6>",
                      "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
         "cpu_baseline": None,
+        "surrogate": sur,
     }
 
 
@@ -241,6 +267,9 @@ def main():
                     help="globule: the measured configuration (1,231 atoms, N2, 10^5 "
                          "trajectories); the others are the batch and large-target runs")
     ap.add_argument("--traj", type=int, default=0, help="trajectories (per target for batch64)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="ellipsoid50k/shell1m: also build the far-field network and time the "
+                         "surrogate path against the plain kernel on the same samples")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the trajectory engine is measured on a single "

@@ -245,6 +245,75 @@ mccs_status mccs_batch_shapes(const mccs_batch *b, int32_t *lanes, double *skin)
 /* Executed work of the last batch run, as mccs_engine_last_work. */
 mccs_status mccs_batch_last_work(mccs_batch *b, uint64_t totals[5]);
 
+/* ------------------------------------------------------------------ */
+/* Far-field surrogate (no counterpart in the reference).  A Fourier-feature
+ * MLP -- 64 features [sin(2 pi B r'), cos(2 pi B r')], r' = (r - centre) *
+ * in_scale, then 3 x 128 SiLU, then 2 outputs (energy of the N2 nitrogen
+ * site and of the N2 charge/dipole site, times out_scale) -- trained on
+ * energies that the exact linked-cell evaluator computes.  BF16 weights,
+ * FP32 accumulation, all products on MFMA.  Parameters are a flat FP32 array
+ * of MCCS_MLP_NPARAM values: W1[64][128], b1[128], W2[128][128], b2[128],
+ * W3[128][128], b3[128], W4[128][2], b4[2] (row-major, W[in][out]). */
+#define MCCS_MLP_NPARAM 41602
+
+typedef struct {
+  float freq[32][3];       /* B */
+  double centre[3];
+  double in_scale;         /* 1 / length scale, 1/A */
+  double out_scale[2];     /* kcal/mol per output unit */
+} mccs_mlp_spec;
+
+/* Outputs y (n x 2, kcal/mol) and, if dy is not NULL, their gradients
+ * (n x 2 x 3, kcal/mol/A) at n points xyz (n x 3). */
+mccs_status mccs_mlp_eval(int32_t device, const mccs_mlp_spec *spec, const float *params,
+                          const double *xyz, uint64_t n, double *y, double *dy);
+
+/* Mean squared error of the scaled outputs against e (n x 2, kcal/mol) over
+ * one batch (n <= 65536) and its gradient with respect to the parameters. */
+mccs_status mccs_mlp_loss_grad(int32_t device, const mccs_mlp_spec *spec, const float *params,
+                               const double *xyz, const double *e, uint64_t n, double *loss,
+                               float *grad);
+
+/* Adam training (cosine-decayed learning rate lr -> lr/20) of params in place
+ * on n points; loss_history (may be NULL) receives one mean loss per epoch. */
+mccs_status mccs_mlp_train(int32_t device, const mccs_mlp_spec *spec, float *params,
+                           const double *xyz, const double *e, uint64_t n, int32_t epochs,
+                           int32_t batch, double lr, uint32_t seed, double *loss_history);
+
+typedef struct {
+  uint64_t n_train;        /* points sampled by the exact evaluator (default 2e6) */
+  int32_t epochs, batch;   /* default 100, 8192 */
+  double lr;               /* default 5e-3 (cosine decay to lr / 20) */
+  double shell;            /* exact force within this distance of any atom (default 8 A) */
+  double blend;            /* exact and surrogate blended over [shell, shell + blend] (2 A) */
+  double grid;             /* spacing of the tabulated network (default 0.25 A) */
+  double feature_length;   /* Fourier frequencies ~ N(0, 1 / feature_length^2) per A (18 A) */
+  uint32_t seed;
+  int32_t table_source;    /* 0: the trained network; 1: the exact evaluator at the grid nodes
+                              (no network; validates the tabulated path) */
+} mccs_surrogate_opts;
+
+typedef struct {
+  double train_loss;                 /* last epoch, scaled units */
+  double val_rmse_u[2], val_rms_u[2];  /* held-out energy error and scale per output, kcal/mol */
+  double val_rmse_f[2], val_rms_f[2];  /* held-out force error and scale per output, kcal/mol/A */
+  double seconds_sample, seconds_train, seconds_table;
+  uint64_t n_train, grid_nodes;
+  mccs_mlp_spec spec;
+} mccs_surrogate_report;
+
+/* Builds the far-field surrogate of an N2 (force type 6) engine's target on
+ * its GPU: samples points outside the shell, evaluates them exactly, trains
+ * the network with the MFMA kernels, and tabulates it on a cubic B-spline
+ * grid that the trajectory kernel interpolates.  params_out (may be NULL)
+ * receives the trained parameters. */
+mccs_status mccs_engine_build_surrogate(mccs_engine *e, const mccs_surrogate_opts *opts,
+                                        mccs_surrogate_report *report, float *params_out);
+
+/* 1: later runs use the surrogate outside the shell (exact force inside it,
+ * blended across the blend band); 0: the exact force everywhere. */
+mccs_status mccs_engine_use_surrogate(mccs_engine *e, int32_t on);
+
 /* Library version string. */
 const char *mccs_version(void);
 

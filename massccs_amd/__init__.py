@@ -64,6 +64,29 @@ class BatchOpts(C.Structure):
                 ("use_graph", C.c_int32), ("skin", C.c_double)]
 
 
+class MlpSpec(C.Structure):
+    """mccs_mlp_spec: Fourier frequencies, input centre/scale, output scales."""
+    _fields_ = [("freq", (C.c_float * 3) * 32), ("centre", C.c_double * 3),
+                ("in_scale", C.c_double), ("out_scale", C.c_double * 2)]
+
+
+class SurrogateOpts(C.Structure):
+    _fields_ = [("n_train", C.c_uint64), ("epochs", C.c_int32), ("batch", C.c_int32),
+                ("lr", C.c_double), ("shell", C.c_double), ("blend", C.c_double),
+                ("grid", C.c_double), ("feature_length", C.c_double), ("seed", C.c_uint32),
+                ("table_source", C.c_int32)]
+
+
+class SurrogateReport(C.Structure):
+    _fields_ = [("train_loss", C.c_double), ("val_rmse_u", C.c_double * 2),
+                ("val_rms_u", C.c_double * 2), ("val_rmse_f", C.c_double * 2),
+                ("val_rms_f", C.c_double * 2), ("seconds_sample", C.c_double),
+                ("seconds_train", C.c_double), ("seconds_table", C.c_double),
+                ("n_train", C.c_uint64), ("grid_nodes", C.c_uint64), ("spec", MlpSpec)]
+
+
+MLP_NPARAM = 41602
+
 RECORD_DTYPE = np.dtype([("hit", "<i4"), ("success", "<i4"), ("tries", "<i4"), ("steps", "<i4"),
                          ("chi", "<f8"), ("domega", "<f8"), ("visits", "<u8"), ("lj", "<u8"),
                          ("coul", "<u8"), ("dip", "<u8")])
@@ -76,7 +99,8 @@ EXPORTS = [
     "mccs_engine_destroy", "mccs_run", "mccs_run_device", "mccs_eval_sites", "mccs_tally",
     "mccs_tally_device", "mccs_version", "mccs_engine_last_work", "mccs_batch_create",
     "mccs_batch_destroy", "mccs_batch_run", "mccs_batch_autotune", "mccs_batch_shapes",
-    "mccs_batch_last_work",
+    "mccs_batch_last_work", "mccs_mlp_eval", "mccs_mlp_loss_grad", "mccs_mlp_train",
+    "mccs_engine_build_surrogate", "mccs_engine_use_surrogate",
 ]
 
 _lib = None
@@ -125,6 +149,17 @@ def lib():
                                       C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.mccs_batch_shapes.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.mccs_batch_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    fp = C.POINTER(C.c_float)
+    L.mccs_mlp_eval.argtypes = [C.c_int32, C.POINTER(MlpSpec), fp, C.POINTER(C.c_double),
+                                C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mccs_mlp_loss_grad.argtypes = [C.c_int32, C.POINTER(MlpSpec), fp, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double), fp]
+    L.mccs_mlp_train.argtypes = [C.c_int32, C.POINTER(MlpSpec), fp, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.c_uint64, C.c_int32, C.c_int32,
+                                 C.c_double, C.c_uint32, C.POINTER(C.c_double)]
+    L.mccs_engine_build_surrogate.argtypes = [C.c_void_p, C.POINTER(SurrogateOpts),
+                                              C.POINTER(SurrogateReport), fp]
+    L.mccs_engine_use_surrogate.argtypes = [C.c_void_p, C.c_int32]
     L.mccs_tally.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                              C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_tally_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
@@ -259,6 +294,24 @@ class Engine:
         _check(lib().mccs_engine_last_work(self.h, out))
         return dict(zip(("visits", "lj", "coul", "dip", "steps"), list(out)))
 
+    def build_surrogate(self, n_train=2_000_000, epochs=100, batch=8192, lr=5e-3, shell=8.0,
+                        blend=2.0, grid=0.25, feature_length=18.0, seed=2104, exact_table=False):
+        """Trains and tabulates the far-field network of this (N2, type 6)
+        engine's target; returns (report dict, trained parameters, spec).
+        exact_table tabulates the exact evaluator instead of the network."""
+        o = SurrogateOpts(n_train, epochs, batch, lr, shell, blend, grid, feature_length, seed,
+                          1 if exact_table else 0)
+        r = SurrogateReport()
+        params = np.zeros(MLP_NPARAM, dtype=np.float32)
+        _check(lib().mccs_engine_build_surrogate(self.h, C.byref(o), C.byref(r),
+                                                 params.ctypes.data_as(C.POINTER(C.c_float))))
+        rep = {k: (list(getattr(r, k)) if k.startswith("val_") else getattr(r, k))
+               for k, _ in SurrogateReport._fields_ if k != "spec"}
+        return rep, params, r.spec
+
+    def use_surrogate(self, on=True):
+        _check(lib().mccs_engine_use_surrogate(self.h, 1 if on else 0))
+
     def eval_sites(self, site, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         f = np.zeros_like(xyz)
@@ -323,6 +376,54 @@ class Batch:
         out = (C.c_uint64 * 5)()
         _check(lib().mccs_batch_last_work(self.h, out))
         return dict(zip(("visits", "lj", "coul", "dip", "steps"), list(out)))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_spec(freq, centre, in_scale, out_scale):
+    sp = MlpSpec()
+    for f in range(32):
+        for d in range(3):
+            sp.freq[f][d] = float(freq[f][d])
+    for d in range(3):
+        sp.centre[d] = float(centre[d])
+    sp.in_scale = float(in_scale)
+    sp.out_scale[0], sp.out_scale[1] = float(out_scale[0]), float(out_scale[1])
+    return sp
+
+
+def mlp_eval(spec, params, xyz, grad=True, device=0):
+    """Network outputs (n, 2) and gradients (n, 2, 3) at points (n, 3), on the GPU."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    y = np.zeros((len(xyz), 2))
+    dy = np.zeros((len(xyz), 2, 3)) if grad else None
+    _check(lib().mccs_mlp_eval(device, C.byref(spec), _fp(params), _dp(xyz), len(xyz), _dp(y),
+                               _dp(dy) if grad else None))
+    return y, dy
+
+
+def mlp_loss_grad(spec, params, xyz, e, device=0):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    e = np.ascontiguousarray(e, dtype=np.float64).reshape(-1, 2)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = C.c_double()
+    _check(lib().mccs_mlp_loss_grad(device, C.byref(spec), _fp(params), _dp(xyz), _dp(e),
+                                    len(xyz), C.byref(loss), _fp(g)))
+    return loss.value, g
+
+
+def mlp_train(spec, params, xyz, e, epochs=10, batch=4096, lr=2e-3, seed=1, device=0):
+    params = np.array(params, dtype=np.float32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    e = np.ascontiguousarray(e, dtype=np.float64).reshape(-1, 2)
+    hist = np.zeros(epochs)
+    _check(lib().mccs_mlp_train(device, C.byref(spec), _fp(params), _dp(xyz), _dp(e), len(xyz),
+                                epochs, batch, lr, seed, _dp(hist)))
+    return params, hist
 
 
 def tally(rec, n_iter, n_traj):

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "mlp.hpp"
 #include "host.hpp"
 #include "massccs.h"
 
@@ -263,6 +264,85 @@ mccs_status mccs_batch_last_work(mccs_batch *b, uint64_t totals[5]) {
   return guard([&] {
     need(b && totals, "null argument");
     mccs::batch_last_work(b->b, totals);
+  });
+}
+
+namespace {
+mccs::MlpSpec to_spec(const mccs_mlp_spec &s) {
+  mccs::MlpSpec m{};
+  static_assert(sizeof(m.freq) == sizeof(s.freq), "frequency table");
+  std::memcpy(m.freq, s.freq, sizeof m.freq);
+  for (int d = 0; d < 3; d++) m.centre[d] = s.centre[d];
+  m.in_scale = s.in_scale;
+  m.out_scale[0] = s.out_scale[0];
+  m.out_scale[1] = s.out_scale[1];
+  return m;
+}
+struct MlpHolder {
+  mccs::Mlp *m;
+  ~MlpHolder() { mccs::mlp_destroy(m); }
+};
+}  // namespace
+
+mccs_status mccs_mlp_eval(int32_t device, const mccs_mlp_spec *spec, const float *params,
+                          const double *xyz, uint64_t n, double *y, double *dy) {
+  return guard([&] {
+    need(spec && params && ((xyz && y) || n == 0), "null argument");
+    if (n == 0) return;
+    MlpHolder h{mccs::mlp_create(device, to_spec(*spec), params, 16384)};
+    mccs::mlp_eval(h.m, xyz, n, y, dy);
+  });
+}
+
+mccs_status mccs_mlp_loss_grad(int32_t device, const mccs_mlp_spec *spec, const float *params,
+                               const double *xyz, const double *e, uint64_t n, double *loss,
+                               float *grad) {
+  return guard([&] {
+    need(spec && params && xyz && e && loss && grad, "null argument");
+    need(n > 0 && n <= 65536, "batch must hold 1..65536 points");
+    MlpHolder h{mccs::mlp_create(device, to_spec(*spec), params, (int)n)};
+    *loss = mccs::mlp_loss_grad(h.m, xyz, e, n, grad);
+  });
+}
+
+mccs_status mccs_mlp_train(int32_t device, const mccs_mlp_spec *spec, float *params,
+                           const double *xyz, const double *e, uint64_t n, int32_t epochs,
+                           int32_t batch, double lr, uint32_t seed, double *loss_history) {
+  return guard([&] {
+    need(spec && params && xyz && e && n > 0 && epochs > 0 && batch > 0, "null or empty argument");
+    need(batch <= 65536, "batch must be at most 65536");
+    const mccs::MlpSpec sp = to_spec(*spec);
+    MlpHolder h{mccs::mlp_create(device, sp, params, batch)};
+    mccs::TrainOpts o;
+    o.epochs = epochs;
+    o.batch = batch;
+    o.lr = lr;
+    o.lr_final = lr / 20.0;
+    o.seed = seed;
+    std::vector<double> hist;
+    mccs::mlp_train_host(h.m, xyz, e, n, o, &hist);
+    mccs::mlp_get_params(h.m, params);
+    if (loss_history)
+      for (size_t i = 0; i < hist.size(); i++) loss_history[i] = hist[i];
+  });
+}
+
+mccs_status mccs_engine_build_surrogate(mccs_engine *e, const mccs_surrogate_opts *opts,
+                                        mccs_surrogate_report *report, float *params_out) {
+  return guard([&] {
+    need(e, "null engine");
+    mccs_surrogate_opts o{};
+    if (opts) o = *opts;
+    mccs_surrogate_report r{};
+    mccs::engine_build_surrogate(e->e, o, r, params_out);
+    if (report) *report = r;
+  });
+}
+
+mccs_status mccs_engine_use_surrogate(mccs_engine *e, int32_t on) {
+  return guard([&] {
+    need(e, "null engine");
+    mccs::engine_use_surrogate(e->e, on != 0);
   });
 }
 

@@ -1,8 +1,11 @@
 // MI355X (gfx950) trajectory engine: per-point site kernel, tally kernels and
 // the host-side launcher behind the C ABI.
 #include "kernels.hpp"
+#include "mlp.hpp"
 
+#include <chrono>
 #include <map>
+#include <random>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -199,6 +202,10 @@ struct Engine {
   std::vector<void *> allocs;
   QueueWS q;
   Tgt *d_tg = nullptr;  // T in device memory (the kernel's target array)
+  // far-field surrogate
+  Mlp *mlp = nullptr;
+  float4 *d_sg = nullptr;
+  bool sur_built = false;
   int spec_below = -1;
   hipStream_t last_stream = nullptr;
   int nsites = 1;
@@ -217,6 +224,8 @@ struct Engine {
   }
   ~Engine() {
     if (device >= 0) hipSetDevice(device);
+    if (mlp) mlp_destroy(mlp);
+    if (d_sg) hipFree(d_sg);
     for (void *p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
   }
@@ -416,12 +425,11 @@ void engine_run_host(Engine *E, const double *u, uint64_t n, mccs_traj_record *r
   hipFree(dr);
 }
 
-void engine_eval_sites(Engine *E, int site, const double *xyz, uint64_t n, double *f, double *U) {
-  if (n == 0) return;
-  HIP_OK(hipSetDevice(E->device));
+// Kernel and probe charge of a site role; fn null for a site without terms.
+static EvalFn site_kernel(const Engine *E, int site, double &q) {
   const int ft = E->ft, gk = E->gk;
   bool lj = false, cen = false, coul = false, dip = false, cut = false;
-  double q = 0.0;
+  q = 0.0;
   if (site == MCCS_SITE_ATOM) {
     if (gk != 1) throw Error(MCCS_ERR_INVALID, "site ATOM needs a one-site probe (He, Ar, co2)");
     lj = true; dip = (ft == 3 || ft == 4); cut = (ft == 2 || ft == 4);
@@ -439,37 +447,54 @@ void engine_eval_sites(Engine *E, int site, const double *xyz, uint64_t n, doubl
   } else {
     throw Error(MCCS_ERR_INVALID, "unknown site role");
   }
-  std::vector<double> hf(3 * n, 0.0), hu(n, 0.0);
-  if (!(lj || coul || dip)) {  // N2 dummy of force types 1/2: no interaction
-    std::memcpy(f, hf.data(), sizeof(double) * 3 * n);
-    std::memcpy(U, hu.data(), sizeof(double) * n);
-    return;
-  }
+  if (!(lj || coul || dip)) return nullptr;  // N2 dummy of force types 1/2: no interaction
   EvalFn fn = select_eval(lj, cen, coul, dip, cut, E->lds);
   if (!fn) throw Error(MCCS_ERR_UNSUPPORTED, "no site kernel for this role");
+  return fn;
+}
+
+// Site forces at device points (exact evaluator), enqueued on the engine's stream.
+static void eval_sites_dev(Engine *E, int site, const double *dx, uint64_t n, double *df, double *du) {
+  double q = 0.0;
+  EvalFn fn = site_kernel(E, site, q);
+  if (!fn) {
+    HIP_OK(hipMemsetAsync(df, 0, sizeof(double) * 3 * n, E->stream));
+    HIP_OK(hipMemsetAsync(du, 0, sizeof(double) * n, E->stream));
+    return;
+  }
   const int block = 256;
   const size_t shmem = lds_bytes(E->T.natoms, E->T.list_cap, block, 64, E->lds, E->gk == 3);
   set_shmem((const void *)fn, shmem);
+  const uint64_t gpb = block / 64;
+  const uint64_t blocks = std::min<uint64_t>((n + gpb - 1) / gpb, (uint64_t)E->cus * 8);
+  Tgt T = E->T;
+  T.sg = nullptr;
+  void *args[] = {&T, &dx, &n, &q, &df, &du};
+  HIP_OK(hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(block), args, shmem, E->stream));
+}
+
+void engine_eval_sites(Engine *E, int site, const double *xyz, uint64_t n, double *f, double *U) {
+  if (n == 0) return;
+  HIP_OK(hipSetDevice(E->device));
   double *dx = nullptr, *df = nullptr, *du = nullptr;
   HIP_OK(hipMalloc(&dx, sizeof(double) * 3 * n));
   HIP_OK(hipMalloc(&df, sizeof(double) * 3 * n));
   HIP_OK(hipMalloc(&du, sizeof(double) * n));
-  HIP_OK(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, E->stream));
-  const uint64_t gpb = block / 64;
-  const uint64_t blocks = std::min<uint64_t>((n + gpb - 1) / gpb, (uint64_t)E->cus * 8);
-  Tgt T = E->T;
-  void *args[] = {&T, &dx, &n, &q, &df, &du};
-  hipError_t err = hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(block), args, shmem,
-                                   E->stream);
-  if (err == hipSuccess) {
-    hipMemcpyAsync(f, df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, E->stream);
-    hipMemcpyAsync(U, du, sizeof(double) * n, hipMemcpyDeviceToHost, E->stream);
-    err = hipStreamSynchronize(E->stream);
+  try {
+    HIP_OK(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, E->stream));
+    eval_sites_dev(E, site, dx, n, df, du);
+    HIP_OK(hipMemcpyAsync(f, df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, E->stream));
+    HIP_OK(hipMemcpyAsync(U, du, sizeof(double) * n, hipMemcpyDeviceToHost, E->stream));
+    HIP_OK(hipStreamSynchronize(E->stream));
+  } catch (...) {
+    hipFree(dx);
+    hipFree(df);
+    hipFree(du);
+    throw;
   }
   hipFree(dx);
   hipFree(df);
   hipFree(du);
-  if (err != hipSuccess) throw Error(MCCS_ERR_DEVICE, std::string("site kernel: ") + hipGetErrorString(err));
 }
 
 void engine_tally_device(Engine *E, const mccs_traj_record *rec_dev, uint64_t n_iter, uint64_t n_traj,
@@ -483,6 +508,383 @@ void engine_tally_device(Engine *E, const mccs_traj_record *rec_dev, uint64_t n_
   HIP_OK(hipGetLastError());
 }
 
+
+
+// ---------------------------------------------------------------- surrogate
+namespace dev {
+
+// Distance from each point to the nearest target atom, capped at cap
+// (cells within ceil(cap / cell) of the point's cell).
+__global__ void mindist_kernel(Tgt T, const float *__restrict__ xyz, uint64_t n, double cap,
+                               float *__restrict__ out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double x = xyz[3 * p], y = xyz[3 * p + 1], z = xyz[3 * p + 2];
+  const int R = (int)ceil(cap / T.cell);
+  const int i = (int)floor((x - T.cx) / T.cell), j = (int)floor((y - T.cy) / T.cell),
+            k = (int)floor((z - T.cz) / T.cell);
+  double best = cap * cap;
+  for (int a = max(0, i - R); a <= min(T.nx - 1, i + R); a++)
+    for (int b = max(0, j - R); b <= min(T.ny - 1, j + R); b++)
+      for (int c = max(0, k - R); c <= min(T.nz - 1, k + R); c++) {
+        const int cell = c + T.nz * b + T.nz * T.ny * a;
+        for (int q = T.cstart[cell]; q < T.cstart[cell + 1]; q++) {
+          const double4 at = T.atoms[q];
+          const double ex = at.x - x, ey = at.y - y, ez = at.z - z;
+          best = fmin(best, ex * ex + ey * ey + ez * ez);
+        }
+      }
+  out[p] = (float)sqrt(best);
+}
+
+// Positions of grid nodes [first, first + n) (x-major).
+__global__ void grid_points_kernel(uint64_t first, uint64_t n, double ox, double oy, double oz,
+                                   double h, int ny, int nz, float *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t g = first + i;
+  const uint64_t ix = g / ((uint64_t)ny * nz), iy = (g / nz) % ny, iz = g % nz;
+  out[3 * i] = (float)(ox + h * ix);
+  out[3 * i + 1] = (float)(oy + h * iy);
+  out[3 * i + 2] = (float)(oz + h * iz);
+}
+
+// Grid node: both energies, tapered to zero between ts and te (distance to
+// the nearest atom), and the distance capped at dcap.
+__global__ void pack_grid_kernel(const float *__restrict__ y, const float *__restrict__ d, uint64_t n,
+                                 float ts, float te, float dcap, float4 *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float di = d[i];
+  float t = 1.f;
+  if (di >= te) t = 0.f;
+  else if (di > ts) {
+    const float x = (di - ts) / (te - ts);
+    t = 1.f - x * x * (3.f - 2.f * x);
+  }
+  out[i] = make_float4(y[2 * i] * t, y[2 * i + 1] * t, fminf(di, dcap), 0.f);
+}
+
+__global__ void f32_to_f64_kernel(const float *__restrict__ a, uint64_t n, double *__restrict__ b) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+__global__ void pair_to_f32_kernel(const double *__restrict__ u0, const double *__restrict__ u1,
+                                   uint64_t n, float *__restrict__ y) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[2 * i] = (float)u0[i];
+  y[2 * i + 1] = (float)u1[i];
+}
+
+__global__ void f64_to_f32_kernel(const double *__restrict__ a, uint64_t n, float *__restrict__ b) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void targets_kernel(const double *__restrict__ uN, const double *__restrict__ uD, uint64_t n,
+                               double s0, double s1, float *__restrict__ e) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  e[2 * i] = (float)(uN[i] / s0);
+  e[2 * i + 1] = (float)(uD[i] / s1);
+}
+
+}  // namespace dev
+
+namespace {
+constexpr double SUR_DCAP = 12.0;  // distances are resolved up to this
+double secs_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+unsigned nblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
+}  // namespace
+
+void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrogate_report &rep,
+                            float *params_out) {
+  if (E->gk != 2 || E->ft != 6)
+    throw Error(MCCS_ERR_UNSUPPORTED, "the far-field surrogate is built for the N2 probe, force type 6");
+  HIP_OK(hipSetDevice(E->device));
+  std::memset(&rep, 0, sizeof rep);
+  const Tgt &T = E->T;
+  const double shell = o.shell > 0 ? o.shell : 8.0, blend = o.blend > 0 ? o.blend : 2.0;
+  const double h = o.grid > 0 ? o.grid : 0.25;
+  const uint64_t ntr = o.n_train ? o.n_train : 2000000ull, nval = 20000;
+  if (shell + blend + 2 * h > SUR_DCAP)
+    throw Error(MCCS_ERR_INVALID, "shell + blend must stay below 11 A");
+  hipStream_t st = E->stream;
+  auto t0 = std::chrono::steady_clock::now();
+
+  // 1. points outside (shell - 1.5 A) of every atom and within the longest
+  //    cutoff of some atom (beyond it the field is exactly zero), uniform
+  const double marg = T.rlong + 1.0;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = T.blo[d] - marg;
+    hi[d] = T.bhi[d] + marg;
+  }
+  const double dmin = shell - 1.5;
+  const uint64_t want = ntr + nval, chunk = 1u << 20;
+  std::vector<double> pts;
+  pts.reserve(3 * want);
+  std::mt19937_64 rng(o.seed ? o.seed : 2104);
+  std::uniform_real_distribution<double> U01(0.0, 1.0);
+  float *dp = nullptr, *dd = nullptr;
+  HIP_OK(hipMalloc(&dp, sizeof(float) * 3 * chunk));
+  HIP_OK(hipMalloc(&dd, sizeof(float) * chunk));
+  std::vector<float> hp(3 * chunk), hd(chunk);
+  std::vector<double> cand(3 * chunk);
+  for (int guard = 0; pts.size() < 3 * want; guard++) {
+    if (guard > 1000) throw Error(MCCS_ERR_INVALID, "no room outside the shell to sample");
+    for (uint64_t i = 0; i < chunk; i++)
+      for (int d = 0; d < 3; d++) {
+        cand[3 * i + d] = lo[d] + (hi[d] - lo[d]) * U01(rng);
+        hp[3 * i + d] = (float)cand[3 * i + d];
+      }
+    HIP_OK(hipMemcpyAsync(dp, hp.data(), sizeof(float) * 3 * chunk, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(mindist_kernel, dim3(nblk(chunk)), dim3(256), 0, st, T, (const float *)dp,
+                       (uint64_t)chunk, T.rlong + 1.0, dd);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(hd.data(), dd, sizeof(float) * chunk, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < chunk && pts.size() < 3 * want; i++)
+      if (hd[i] >= dmin && hd[i] < T.rlong)  // outside the shell, inside the cutoff
+        for (int d = 0; d < 3; d++) pts.push_back((double)hp[3 * i + d]);  // the float point itself
+  }
+  hipFree(dp);
+  hipFree(dd);
+  // exact energies (and forces, for validation) of both site roles
+  double *dx = nullptr, *fN = nullptr, *uN = nullptr, *fD = nullptr, *uD = nullptr;
+  HIP_OK(hipMalloc(&dx, sizeof(double) * 3 * want));
+  HIP_OK(hipMalloc(&fN, sizeof(double) * 3 * want));
+  HIP_OK(hipMalloc(&uN, sizeof(double) * want));
+  HIP_OK(hipMalloc(&fD, sizeof(double) * 3 * want));
+  HIP_OK(hipMalloc(&uD, sizeof(double) * want));
+  HIP_OK(hipMemcpyAsync(dx, pts.data(), sizeof(double) * 3 * want, hipMemcpyHostToDevice, st));
+  eval_sites_dev(E, MCCS_SITE_N, dx, want, fN, uN);
+  eval_sites_dev(E, MCCS_SITE_DUMMY, dx, want, fD, uD);
+  std::vector<double> hu[2], hf[2];
+  for (int r = 0; r < 2; r++) {
+    hu[r].resize(want);
+    hf[r].resize(3 * want);
+  }
+  HIP_OK(hipMemcpyAsync(hu[0].data(), uN, sizeof(double) * want, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(hu[1].data(), uD, sizeof(double) * want, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(hf[0].data(), fN, sizeof(double) * 3 * want, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(hf[1].data(), fD, sizeof(double) * 3 * want, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  rep.seconds_sample = secs_since(t0);
+
+  // 2. network: scales from the data, Fourier frequencies N(0, (L / flen)^2)
+  //    in units of the scaled input (1 / flen per A)
+  MlpSpec spec{};
+  double L = 0.0;
+  for (int d = 0; d < 3; d++) {
+    spec.centre[d] = 0.5 * (lo[d] + hi[d]);
+    L = std::max(L, 0.5 * (hi[d] - lo[d]));
+  }
+  spec.in_scale = 1.0 / L;
+  for (int r = 0; r < 2; r++) {
+    double ss = 0.0;
+    for (uint64_t i = 0; i < ntr; i++) ss += hu[r][i] * hu[r][i];
+    spec.out_scale[r] = std::max(std::sqrt(ss / (double)ntr), 1e-6);
+  }
+  std::normal_distribution<double> N01(0.0, 1.0);
+  const double flen = o.feature_length > 0 ? o.feature_length : 18.0;
+  for (int f = 0; f < MLP_NF; f++)
+    for (int d = 0; d < 3; d++) spec.freq[f][d] = (float)(N01(rng) * L / flen);
+  std::vector<float> params(MLP_NPARAM, 0.0f);
+  auto init = [&](size_t off, int fin, int fout) {  // uniform +-sqrt(6 / (fin + fout))
+    const double a = std::sqrt(6.0 / (fin + fout));
+    for (size_t i = 0; i < (size_t)fin * fout; i++) params[off + i] = (float)(a * (2.0 * U01(rng) - 1.0));
+  };
+  init(MLP_W1, MLP_IN, MLP_H);
+  init(MLP_W2, MLP_H, MLP_H);
+  init(MLP_W3, MLP_H, MLP_H);
+  init(MLP_W4, MLP_H, MLP_OUT);
+  if (E->mlp) mlp_destroy(E->mlp);
+  E->mlp = nullptr;
+  E->mlp = mlp_create(E->device, spec, params.data(), 65536);
+
+  // 3. train on the first ntr points (device-resident FP32 positions and
+  //    scaled energies)
+  auto t1 = std::chrono::steady_clock::now();
+  float *txyz = nullptr, *te = nullptr, *tdir = nullptr, *tder = nullptr;
+  HIP_OK(hipMalloc(&txyz, sizeof(float) * 3 * ntr));
+  HIP_OK(hipMalloc(&te, sizeof(float) * 2 * ntr));
+  HIP_OK(hipMalloc(&tdir, sizeof(float) * 3 * ntr));
+  HIP_OK(hipMalloc(&tder, sizeof(float) * 2 * ntr));
+  double wd = 1.0;
+  {  // a random direction per point and the exact directional derivatives
+     // -F . dir of both outputs (scaled), matched by central differences
+    std::vector<float> dir(3 * ntr), der(2 * ntr);
+    double ss = 0.0;
+    for (uint64_t i = 0; i < ntr; i++) {
+      double v[3], nn = 0.0;
+      do {
+        nn = 0.0;
+        for (int d = 0; d < 3; d++) {
+          v[d] = N01(rng);
+          nn += v[d] * v[d];
+        }
+      } while (nn < 1e-12);
+      nn = std::sqrt(nn);
+      for (int d = 0; d < 3; d++) dir[3 * i + d] = (float)(v[d] / nn);
+      for (int r = 0; r < 2; r++) {
+        double dd = 0.0;
+        for (int d = 0; d < 3; d++) dd -= hf[r][3 * i + d] * (double)dir[3 * i + d];
+        der[2 * i + r] = (float)(dd / spec.out_scale[r]);
+        ss += (double)der[2 * i + r] * der[2 * i + r];
+      }
+    }
+    wd = 2.0 * ntr / std::max(ss, 1e-30);  // both terms of comparable size
+    HIP_OK(hipMemcpy(tdir, dir.data(), sizeof(float) * 3 * ntr, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(tder, der.data(), sizeof(float) * 2 * ntr, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3(nblk(3 * ntr)), dim3(256), 0, st, (const double *)dx,
+                     (uint64_t)(3 * ntr), txyz);
+  hipLaunchKernelGGL(targets_kernel, dim3(nblk(ntr)), dim3(256), 0, st, (const double *)uN,
+                     (const double *)uD, ntr, spec.out_scale[0], spec.out_scale[1], te);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(st));
+  TrainOpts to;
+  to.epochs = o.epochs > 0 ? o.epochs : 100;
+  to.batch = o.batch > 0 ? o.batch : 8192;
+  to.lr = o.lr > 0 ? o.lr : 5e-3;
+  to.lr_final = to.lr / 20.0;
+  to.seed = o.seed ? o.seed : 2104;
+  to.d_dir = tdir;
+  to.d_deriv = tder;
+  to.fd_h = 0.2;
+  to.w_deriv = wd;
+  rep.train_loss = mlp_train(E->mlp, txyz, te, ntr, to, nullptr);
+  hipFree(txyz);
+  hipFree(te);
+  hipFree(tdir);
+  hipFree(tder);
+  hipFree(dx);
+  hipFree(fN);
+  hipFree(uN);
+  hipFree(fD);
+  hipFree(uD);
+  rep.seconds_train = secs_since(t1);
+
+  // 4. held-out error of energies and forces (F = -grad U)
+  {
+    std::vector<double> y(2 * nval), dy(6 * nval);
+    mlp_eval(E->mlp, pts.data() + 3 * ntr, nval, y.data(), dy.data());
+    for (int r = 0; r < 2; r++) {
+      double eu = 0, su = 0, ef = 0, sf = 0;
+      for (uint64_t i = 0; i < nval; i++) {
+        const double ue = hu[r][ntr + i];
+        eu += (y[2 * i + r] - ue) * (y[2 * i + r] - ue);
+        su += ue * ue;
+        for (int d = 0; d < 3; d++) {
+          const double fe = hf[r][3 * (ntr + i) + d], fm = -dy[6 * i + 3 * r + d];
+          ef += (fm - fe) * (fm - fe);
+          sf += fe * fe;
+        }
+      }
+      rep.val_rmse_u[r] = std::sqrt(eu / nval);
+      rep.val_rms_u[r] = std::sqrt(su / nval);
+      rep.val_rmse_f[r] = std::sqrt(ef / (3.0 * nval));
+      rep.val_rms_f[r] = std::sqrt(sf / (3.0 * nval));
+    }
+  }
+
+  // 5. tabulate: energies and capped nearest-atom distance on the grid
+  auto t2 = std::chrono::steady_clock::now();
+  const double gm = T.rlong + 3.0 * h;
+  double go[3];
+  int gn[3];
+  for (int d = 0; d < 3; d++) {
+    go[d] = T.blo[d] - gm;
+    gn[d] = (int)std::ceil((T.bhi[d] - T.blo[d] + 2.0 * gm) / h) + 1;
+  }
+  const uint64_t nodes = (uint64_t)gn[0] * gn[1] * gn[2];
+  if (E->d_sg) hipFree(E->d_sg);
+  E->d_sg = nullptr;
+  HIP_OK(hipMalloc(&E->d_sg, sizeof(float4) * nodes));
+  const uint64_t gch = 1u << 20;
+  float *gp = nullptr, *gd = nullptr, *gy = nullptr;
+  double *g64 = nullptr, *gf = nullptr, *gu0 = nullptr, *gu1 = nullptr;
+  HIP_OK(hipMalloc(&gp, sizeof(float) * 3 * gch));
+  HIP_OK(hipMalloc(&gd, sizeof(float) * gch));
+  HIP_OK(hipMalloc(&gy, sizeof(float) * 2 * gch));
+  const bool exact_src = o.table_source == 1;
+  if (exact_src) {
+    HIP_OK(hipMalloc(&g64, sizeof(double) * 3 * gch));
+    HIP_OK(hipMalloc(&gf, sizeof(double) * 3 * gch));
+    HIP_OK(hipMalloc(&gu0, sizeof(double) * gch));
+    HIP_OK(hipMalloc(&gu1, sizeof(double) * gch));
+  }
+  // the network is tapered to zero over the last 4 A before the cutoff so
+  // that, splines included, it vanishes where the exact field does
+  const float tap0 = exact_src ? 1e30f : (float)(T.rlong - 5.0);
+  const float tap1 = exact_src ? 2e30f : (float)(T.rlong - 1.0);
+  for (uint64_t f0 = 0; f0 < nodes; f0 += gch) {
+    const uint64_t c = std::min(gch, nodes - f0);
+    hipLaunchKernelGGL(grid_points_kernel, dim3(nblk(c)), dim3(256), 0, st, f0, c, go[0], go[1], go[2], h,
+                       gn[1], gn[2], gp);
+    hipLaunchKernelGGL(mindist_kernel, dim3(nblk(c)), dim3(256), 0, st, T, (const float *)gp, c,
+                       T.rlong + 1.0, gd);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    if (exact_src) {  // the exact evaluator at the nodes (validation of the tabulated path)
+      hipLaunchKernelGGL(f32_to_f64_kernel, dim3(nblk(3 * c)), dim3(256), 0, st, (const float *)gp,
+                         (uint64_t)(3 * c), g64);
+      eval_sites_dev(E, MCCS_SITE_N, g64, c, gf, gu0);
+      eval_sites_dev(E, MCCS_SITE_DUMMY, g64, c, gf, gu1);
+      hipLaunchKernelGGL(pair_to_f32_kernel, dim3(nblk(c)), dim3(256), 0, st, (const double *)gu0,
+                         (const double *)gu1, c, gy);
+    } else {
+      mlp_eval_device(E->mlp, gp, c, gy, nullptr);
+    }
+    hipLaunchKernelGGL(pack_grid_kernel, dim3(nblk(c)), dim3(256), 0, st, (const float *)gy,
+                       (const float *)gd, c, tap0, tap1, (float)SUR_DCAP, E->d_sg + f0);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipStreamSynchronize(st));
+  hipFree(gp);
+  hipFree(gd);
+  hipFree(gy);
+  if (exact_src) {
+    hipFree(g64);
+    hipFree(gf);
+    hipFree(gu0);
+    hipFree(gu1);
+  }
+  rep.seconds_table = secs_since(t2);
+  rep.n_train = ntr;
+  rep.grid_nodes = nodes;
+  for (int f = 0; f < MLP_NF; f++)
+    for (int d = 0; d < 3; d++) rep.spec.freq[f][d] = spec.freq[f][d];
+  for (int d = 0; d < 3; d++) rep.spec.centre[d] = spec.centre[d];
+  rep.spec.in_scale = spec.in_scale;
+  rep.spec.out_scale[0] = spec.out_scale[0];
+  rep.spec.out_scale[1] = spec.out_scale[1];
+  if (params_out) mlp_get_params(E->mlp, params_out);
+
+  E->T.sox = go[0];
+  E->T.soy = go[1];
+  E->T.soz = go[2];
+  E->T.sih = 1.0 / h;
+  E->T.snx = gn[0];
+  E->T.sny = gn[1];
+  E->T.snz = gn[2];
+  E->T.sd0 = shell;
+  E->T.sd1 = shell + blend;
+  E->T.sdcap = SUR_DCAP;
+  E->sur_built = true;
+}
+
+void engine_use_surrogate(Engine *E, bool on) {
+  HIP_OK(hipSetDevice(E->device));
+  if (on && !E->sur_built) throw Error(MCCS_ERR_INVALID, "no surrogate has been built for this engine");
+  E->T.sg = on ? E->d_sg : nullptr;
+  HIP_OK(hipMemcpy(E->d_tg, &E->T, sizeof(Tgt), hipMemcpyHostToDevice));
+}
 
 // ---------------------------------------------------------------- batches
 // Many targets in one run: targets are grouped by probe kind and force type

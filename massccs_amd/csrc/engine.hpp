@@ -28,6 +28,11 @@ void engine_tally_device(Engine *e, const mccs_traj_record *rec_dev, uint64_t n_
                          uint64_t n_traj, double *omega_dev, int64_t *counts_dev,
                          double *mean_err_dev, void *stream);
 
+// Far-field surrogate of an N2 type-6 engine (trains and tabulates on the GPU).
+void engine_build_surrogate(Engine *e, const mccs_surrogate_opts &o, mccs_surrogate_report &rep,
+                            float *params_out);
+void engine_use_surrogate(Engine *e, bool on);
+
 // Several targets in one run (one persistent launch per probe kind / force
 // type, captured in a hipGraph).
 struct Batch;

@@ -75,6 +75,14 @@ struct Tgt {
   double rfar2;             // no term reaches beyond this squared distance
   double rlong;             // longest cutoff of the run's terms
   double skin;              // Verlet-list skin (A)
+  // far-field surrogate (N2, force type 6): cubic B-spline coefficients of
+  // the network's two site energies and of the distance to the nearest atom
+  // (x = E_N, y = E_dummy, z = d) on a grid; null when not in use
+  const float4 *sg;
+  double sox, soy, soz, sih;  // grid origin and 1 / spacing
+  int snx, sny, snz;
+  double sd0, sd1;          // exact below sd0, blended to the surrogate at sd1
+  double sdcap;             // distance stored for nodes at or beyond the cap
 };
 
 // ---------------------------------------------------------------- probes
@@ -880,6 +888,7 @@ struct GState {
   unsigned long long j;         // trajectory (index over all targets of the launch)
   int t;                        // its target
   int k;                        // try (restart) index of the trajectory, 0..TRIES_MAX-1
+  double sur[3][8];             // surrogate per site: U, grad U, d, grad d
   unsigned long long slot;      // spawn slot taken while its item is not yet written
   int steps;                    // steps of this try
   int cont;                     // 1: the group goes on with try k of trajectory j
@@ -1001,6 +1010,64 @@ __device__ void stage(const Tgt &T, Shm &sh, char *smem) {
   sh.base = scratch + 2 * L * g + L;
   sh.state = states + g;
   sh.list = lists + sizeof(LIdx<LDS>) * (size_t)T.list_cap * g;
+}
+
+// ---------------------------------------------------------------- surrogate
+// Uniform cubic B-spline weights of the 4 nodes around t in [0, 1) and their
+// derivatives.
+__device__ __forceinline__ void bsw(double t, double w[4], double dw[4]) {
+  const double u = 1.0 - t, t2 = t * t, t3 = t2 * t;
+  w[0] = u * u * u * (1.0 / 6.0);
+  w[1] = (3.0 * t3 - 6.0 * t2 + 4.0) * (1.0 / 6.0);
+  w[2] = (-3.0 * t3 + 3.0 * t2 + 3.0 * t + 1.0) * (1.0 / 6.0);
+  w[3] = t3 * (1.0 / 6.0);
+  dw[0] = -0.5 * u * u;
+  dw[1] = 1.5 * t2 - 2.0 * t;
+  dw[2] = -1.5 * t2 + t + 0.5;
+  dw[3] = 0.5 * t2;
+}
+
+// Surrogate energy of one site (role 0: N, 1: dummy) and the distance field,
+// with gradients, at p; the 64 spline nodes are spread over the group.
+template <int L>
+__device__ void spline_site(const Tgt &T, int lane, const double p[3], int role, double &U,
+                            double gU[3], double &d, double gd[3]) {
+  const double ux = (p[0] - T.sox) * T.sih, uy = (p[1] - T.soy) * T.sih, uz = (p[2] - T.soz) * T.sih;
+  const double fx = floor(ux), fy = floor(uy), fz = floor(uz);
+  double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+  bsw(ux - fx, wx, dx);
+  bsw(uy - fy, wy, dy);
+  bsw(uz - fz, wz, dz);
+  const long ix0 = (long)fx - 1, iy0 = (long)fy - 1, iz0 = (long)fz - 1;
+  double aU = 0, ax = 0, ay = 0, az = 0, ad = 0, bx = 0, by = 0, bz = 0;
+  for (int nd = lane; nd < 64; nd += L) {
+    const int a = nd >> 4, b = (nd >> 2) & 3, c = nd & 3;
+    const long ix = ix0 + a, iy = iy0 + b, iz = iz0 + c;
+    double v = 0.0, dv = T.sdcap;
+    if (ix >= 0 && ix < T.snx && iy >= 0 && iy < T.sny && iz >= 0 && iz < T.snz) {
+      const float4 g = T.sg[((size_t)ix * T.sny + iy) * T.snz + iz];
+      v = role ? (double)g.y : (double)g.x;
+      dv = (double)g.z;
+    }
+    const double w = wx[a] * wy[b] * wz[c];
+    const double gx = dx[a] * wy[b] * wz[c], gy = wx[a] * dy[b] * wz[c], gz = wx[a] * wy[b] * dz[c];
+    aU += w * v;
+    ax += gx * v;
+    ay += gy * v;
+    az += gz * v;
+    ad += w * dv;
+    bx += gx * dv;
+    by += gy * dv;
+    bz += gz * dv;
+  }
+  U = gsum<L>(aU);
+  gU[0] = gsum<L>(ax) * T.sih;
+  gU[1] = gsum<L>(ay) * T.sih;
+  gU[2] = gsum<L>(az) * T.sih;
+  d = gsum<L>(ad);
+  gd[0] = gsum<L>(bx) * T.sih;
+  gd[1] = gsum<L>(by) * T.sih;
+  gd[2] = gsum<L>(bz) * T.sih;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -1302,7 +1369,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #pragma unroll
       for (int s = 0; s < S::NS; s++)
         for (int d = 0; d < 3; d++) p[s][d] = st.pos[s][d];
-      evaluate<S, L, LDS, COUNT>(T, sh, lane, p, qs, f, U, w, &st.vl);
+      bool exact = true;
+      if constexpr (GK == 2 && FT == 6) {
+        if (T.sg) {
+          // far-field surrogate: exact evaluation only when a site is closer
+          // to the target than the end of the blend band (values kept in LDS
+          // across the exact evaluation)
+          exact = false;
+#pragma unroll
+          for (int s = 0; s < S::NS; s++) {
+            double u, gu[3], dd, gd[3];
+            spline_site<L>(T, lane, p[s], s == 2 ? 1 : 0, u, gu, dd, gd);
+            exact = exact || dd < T.sd1;
+            if (lane == 0) {
+              st.sur[s][0] = u;
+              st.sur[s][4] = dd;
+              for (int d = 0; d < 3; d++) {
+                st.sur[s][1 + d] = gu[d];
+                st.sur[s][5 + d] = gd[d];
+              }
+            }
+          }
+          lds_fence();
+        }
+      }
+      if (exact) evaluate<S, L, LDS, COUNT>(T, sh, lane, p, qs, f, U, w, &st.vl);
+      if constexpr (GK == 2 && FT == 6) {
+        if (T.sg) {
+#pragma unroll
+          for (int s = 0; s < S::NS; s++) {
+            const double su = st.sur[s][0], sdd = st.sur[s][4];
+            if (sdd >= T.sd1) {  // surrogate only
+              U[s] = su;
+              for (int d = 0; d < 3; d++) f[s][d] = -st.sur[s][1 + d];
+            } else if (sdd > T.sd0) {  // blend: U = S Uex + (1 - S) Usur
+              const double wdt = T.sd1 - T.sd0, t = (sdd - T.sd0) / wdt;
+              const double Sw = 1.0 - t * t * (3.0 - 2.0 * t);
+              const double dS = -6.0 * t * (1.0 - t) / wdt;
+              const double dU = U[s] - su;
+              for (int d = 0; d < 3; d++)
+                f[s][d] = Sw * f[s][d] - (1.0 - Sw) * st.sur[s][1 + d] - dU * dS * st.sur[s][5 + d];
+              U[s] = Sw * U[s] + (1.0 - Sw) * su;
+            }
+          }
+        }
+      }
       double Ut = 0.0;
 #pragma unroll
       for (int s = 0; s < S::NS; s++) Ut += U[s];
