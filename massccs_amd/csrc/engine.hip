@@ -4,6 +4,7 @@
 #include "mlp.hpp"
 
 #include <chrono>
+#include <deque>
 #include <map>
 #include <random>
 #include <memory>
@@ -126,6 +127,10 @@ constexpr uint64_t SLOT_SLACK = 8192;  // spawn slots beyond 8 per trajectory
 // 256-byte line (ticket, shead, stail, active, work[5], stalled), and a
 // per-trajectory workspace grown on demand.
 struct QueueWS {
+  // owns device memory: never copied
+  QueueWS() = default;
+  QueueWS(const QueueWS &) = delete;
+  QueueWS &operator=(const QueueWS &) = delete;
   unsigned long long *qctl = nullptr;
   char *qws = nullptr;
   uint64_t qcap = 0;
@@ -929,7 +934,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   std::vector<Engine *> eng;  // per target: device-resident target data
   std::vector<std::string> sig;
-  std::vector<BatchClass> cls;
+  std::deque<BatchClass> cls;  // elements stay in place (they own device memory)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint64_t> gkey;

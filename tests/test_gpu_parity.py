@@ -153,6 +153,41 @@ def test_trajectories_globule_n2(workdir):
     assert ok_big.mean() >= 0.95 and ok_small.all()
 
 
+@pytest.mark.parametrize("gas,n,lanes", [("Ar", 1000, 16), ("CO2", 200, 64), ("co2", 600, 16)])
+def test_trajectories_other_probes(gas, n, lanes):
+    """Ar (He integrator with the Ar mass and mixed LJ), the rigid 3-site CO2
+    (Ciccotti forces and the reference's vcm quirk) and the 1-site co2."""
+    cfg = dict(targetFileName=C60_XYZ if gas != "CO2" else UBQ_PQR, GasBuffer=gas,
+               polarizability="yes", seed=2104, **LONG)
+    print("case", gas, flush=True)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    u = P.samples(n)
+    ref = orc.run(u)
+    rec = M.Engine(P, lanes=lanes).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, gas)
+    assert flow >= 0.98
+    assert ok_big.mean() >= 0.95 and ok_small.all()
+
+
+def test_trajectories_global_memory_atoms(workdir):
+    """The path large targets take: atoms read from HBM (not staged in LDS),
+    32-bit neighbour lists."""
+    cfg = globule_cfg(workdir)
+    print("case globule_N2 global atoms", flush=True)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    u = P.samples(200)
+    ref = orc.run(u)
+    rec = M.Engine(P, use_lds=0).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "globule_N2_global")
+    assert flow >= 0.98
+    assert ok_big.mean() >= 0.95 and ok_small.all()
+    # identical to the LDS-staged run bit for bit (same lists, same order)
+    lds = M.Engine(P, use_lds=1).run(u)
+    assert np.array_equal(lds.view(np.uint8), rec.view(np.uint8))
+
+
 def test_tally_device_matches_host():
     import torch
     cfg = dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", seed=2104, **LONG)
