@@ -35,6 +35,11 @@ FP64_PEAK_TFLOPS = 78.6
 # LJ term (18); shifted Coulomb (19); smoothed induced dipole field and
 # gradient (39); the dipole epilogue per site evaluation (27).
 W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
+# HBM bytes per launch of this workload's trajectory kernel, from two
+# rocprofv3 --pmc runs: FETCH_SIZE 27,120 KiB x 2.0 + WRITE_SIZE 33,600 KiB
+# x 1.0, each scaled by what the counter reads for a 1 GiB copy
+# (profiles/traj_kernel_globule_n2_counters.md); not measured by this script.
+TRAFFIC_BYTES = (27120 * 2 + 33600) * 1024
 
 
 def workload_cfg(tmp):
@@ -353,7 +358,7 @@ def main():
                              "executed": int(executed["steps"])},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "traj_kernel<N2, type 6>",
+                     "traffic": TRAFFIC_BYTES, "kernel": "traj_kernel<N2, type 6>",
                      "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
     }
     if not args.no_cpu_baseline:

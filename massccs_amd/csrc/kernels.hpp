@@ -612,7 +612,17 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       const int n = vl->n;
       if (vl->valid > 0) {
         const LIdx<LDS> *lst = static_cast<const LIdx<LDS> *>(sh.list);
-        for (int t = lane; t < n; t += L) visit((int)lst[t], 1);
+        // software-pipelined: the next atom's data is read while this one's
+        // terms are computed (hides the LDS/L2 latency of the gather)
+        double4 at = make_double4(0.0, 0.0, 0.0, 0.0);
+        double2 lp = make_double2(0.0, 0.0), lc = make_double2(0.0, 0.0);
+        if (lane < n) fetch<LDS, CEN>(T, sh, (int)lst[lane], at, lp, lc);
+        for (int t = lane; t < n; t += L) {
+          const double4 cat = at;
+          const double2 clp = lp, clc = lc;
+          if (t + L < n) fetch<LDS, CEN>(T, sh, (int)lst[t + L], at, lp, lc);
+          pair_terms<S, COUNT>(T, p, qs, cat, clp, clc, 1, a, w);
+        }
         reduce<S, L>(a);
         finish<S>(T, a, -1, f, U);
         return;
