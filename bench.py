@@ -278,7 +278,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the trajectory engine is measured on a single "
-              "MI355X (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "MI355X; trajectories would shard by index range across GPUs, which is not run here", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch64":
         print(json.dumps(run_batch(args)))
