@@ -1,0 +1,49 @@
+"""The drop-in command line on the GPU against the reference's published
+logs: same report structure, per-iteration counts and CCS (10 x 10^4
+trajectories, seed 2104)."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from common import GOLDEN, M, PUBLISHED, gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+def _run_cli(tmp_path, key):
+    cfg = dict(PUBLISHED[key])
+    p = tmp_path / "input.json"
+    p.write_text(json.dumps(cfg))
+    r = subprocess.run([M.CLI_PATH, str(p)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@pytest.mark.parametrize("key", ["c60_He", "ubq_He", "c60_Ar"])
+def test_cli_reproduces_published_ccs(tmp_path, key):
+    print("case cli", key, flush=True)
+    out = _run_cli(tmp_path, key)
+    with open(os.path.join(GOLDEN, "published.json")) as fh:
+        pub = json.load(fh)[key]
+    omega = [float(x) for x in re.findall(r"^omega: (\S+)$", out, re.M)]
+    nfree = [int(x) for x in re.findall(r"^Nfree: (\d+)$", out, re.M)]
+    nscat = [int(x) for x in re.findall(r"^Nscatter: (\d+)$", out, re.M)]
+    mean = float(re.search(r"average value of CCS = (\S+) Ang\^2", out).group(1))
+    err = float(re.search(r"error value of CCS = (\S+) Ang\^2", out).group(1))
+    assert len(omega) == 10 and "Program finished..." in out and "CCS time:" in out
+    # misses depend only on the samples and the ellipsoid: identical
+    assert nfree == pub["nfree"]
+    # scattered counts and CCS: within 0.1 % and within the published standard error
+    assert np.allclose(nscat, pub["nscatter"], atol=3)
+    print(key, "ccs", mean, err, "published", pub["mean"], pub["err"], "omega", omega, flush=True)
+    assert abs(mean / pub["mean"] - 1) <= 1e-3
+    assert abs(mean - pub["mean"]) <= pub["err"]
+    assert abs(err / pub["err"] - 1) <= 0.05
+    # in fact the printed CCS and error are the published ones, digit for digit
+    assert mean == pub["mean"] and err == pub["err"]
+    assert np.allclose(omega, pub["omega"], rtol=1e-5, atol=0)
