@@ -51,7 +51,7 @@ typedef struct {
   uint32_t n_probe, n_iter, seed, nthreads;
   double dt, temperature, skin, lj_cutoff, coul_cutoff, alpha;
   int32_t gas;              /* MCCS_GAS_* */
-  int32_t equipotential;    /* must be 0: the equipotential ellipsoid is not built */
+  int32_t equipotential;    /* 1: equipotential ellipsoid (src/Equipotential.cpp), computed on the GPU */
   int32_t short_range_cutoff, long_range_flag, long_range_cutoff, polarizability;
   int32_t user_ff;          /* 1 if force_field is set */
   int32_t alpha_set;        /* 1 if alpha carries a value */

@@ -13,6 +13,7 @@
 #include <random>
 #include <sstream>
 
+#include "equipot.hpp"
 #include "host.hpp"
 
 namespace mccs {
@@ -563,8 +564,6 @@ Prepared *prepare(const mccs_config &cfg, const std::string &gas_str, int64_t n_
   auto secs = [](clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double>(b - a).count();
   };
-  if (cfg.equipotential)
-    throw Error(MCCS_ERR_UNSUPPORTED, "Equipotential ellipsoid is not implemented; use \"Equipotential\": \"no\"");
   if (cfg.gas < 1 || cfg.gas > 5) throw Error(MCCS_ERR_INVALID, "unknown gas buffer");
   if (!(cfg.dt > 0) || !(cfg.temperature > 0) || !(cfg.lj_cutoff > 0) || !(cfg.skin >= 0) ||
       !(cfg.coul_cutoff > 0))
@@ -645,14 +644,32 @@ Prepared *prepare(const mccs_config &cfg, const std::string &gas_str, int64_t n_
   P->alpha = (cfg.alpha_set ? cfg.alpha : 0.0) * ALPHA_TO_KCAL_MOL;
 
   auto t2 = clk::now();
-  ellipsoid(t, *P);
-  rep << "*********************************************************\n";
-  rep << "Geometric Ellipsoid: \n";
-  rep << "*********************************************************\n";
-  rep << "maximal distances: " << std::abs(P->maxX) << "  " << std::abs(P->maxY) << "  "
-      << std::abs(P->maxZ) << "  Ang\n";
-  rep << "Initial axis length: " << P->a0 << "  " << P->b0 << "  " << P->c0 << "  Ang\n";
-  rep << "Ellipsoid axis length: " << P->a << "  " << P->b << "  " << P->c << "  Ang\n";
+  if (cfg.equipotential) {  // src/System.cpp:120-125
+    EquiInput in;
+    in.gas = cfg.gas;
+    in.polarizability = cfg.polarizability;
+    in.temperature = cfg.temperature;
+    in.mu = P->mu;
+    in.alpha = P->alpha;
+    in.maxX = P->maxX; in.maxY = P->maxY; in.maxZ = P->maxZ;
+    in.x = t.x; in.y = t.y; in.z = t.z; in.q = t.q;
+    in.eps = t.eps; in.sig = t.sig; in.eps_c = t.eps_c; in.sig_c = t.sig_c;
+    const EquiResult r = equipotential_ellipsoid(in);
+    P->a = r.a; P->b = r.b; P->c = r.c;
+    P->a0 = r.a; P->b0 = r.b; P->c0 = r.c;
+    P->bmax = std::max(std::max(r.a, r.b), r.c);
+    rep << r.report;
+    rep << "ellipsoid axis length: " << P->a << "  " << P->b << "  " << P->c << "  Ang\n";
+  } else {
+    ellipsoid(t, *P);
+    rep << "*********************************************************\n";
+    rep << "Geometric Ellipsoid: \n";
+    rep << "*********************************************************\n";
+    rep << "maximal distances: " << std::abs(P->maxX) << "  " << std::abs(P->maxY) << "  "
+        << std::abs(P->maxZ) << "  Ang\n";
+    rep << "Initial axis length: " << P->a0 << "  " << P->b0 << "  " << P->c0 << "  Ang\n";
+    rep << "Ellipsoid axis length: " << P->a << "  " << P->b << "  " << P->c << "  Ang\n";
+  }
   auto t3 = clk::now();
   rep << "ellipsoid calculation time: " << fmt_time(secs(t2, t3)) << " s\n";
   rep << "maximal impact parameter: " << P->bmax << " Ang\n";

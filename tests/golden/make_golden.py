@@ -12,8 +12,10 @@ own code paths, driven through the harness:
   published.json     the reference's published per-iteration results
                      (ccs.out, ccs_He.log, ccs_N2.log, ccs_Ar.log), parsed
                      from those files
+  equi_*.json        equipotential-ellipsoid axes (src/Equipotential.cpp),
+                     with the reference's OpenMP reductions on one thread
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [equi]
 """
 import hashlib
 import json
@@ -91,6 +93,25 @@ def parse_log(path):
     return dict(omega=om, nfree=fr, nscatter=sc, nlost=lo, mean=mean, err=err)
 
 
+EQUI = {
+    "c60_He_t4": dict(CONFIGS["c60_He_t4"], Equipotential="yes"),
+    "ubq_He_t4": dict(CONFIGS["ubq_He_t4"], Equipotential="yes"),
+    "ubq_N2_t6": dict(CONFIGS["ubq_N2_t6"], Equipotential="yes"),
+    "c60_Ar_t4": dict(targetFileName=C60_XYZ, GasBuffer="Ar", polarizability="yes", seed=2104,
+                      Equipotential="yes", **LONG),
+}
+
+
+def equi():
+    os.environ["OMP_NUM_THREADS"] = "1"  # the reference sums its potentials with OpenMP reductions
+    for name, cfg in EQUI.items():
+        prep = O.harness_prep(cfg)
+        with open(os.path.join(HERE, "equi_%s.json" % name), "w") as fh:
+            json.dump({"config": rel(cfg), "abc": prep["abc"], "bmax": prep["bmax"],
+                       "grid": prep["grid"], "halfbox": prep["halfbox"]}, fh, indent=1)
+        print(name, prep["abc"], flush=True)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_harness not built (make -C oracle ref)")
@@ -136,4 +157,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "equi":
+        equi()
+    else:
+        main()

@@ -269,6 +269,8 @@ def parse_prep(text):
         if not t:
             continue
         k, v = t[0], t[1:]
+        if k == "equipotential":  # the reference's destructor message
+            continue
         if k == "atom":
             d["atoms"].append([float(x) for x in v])
         elif k == "cell":

@@ -203,3 +203,11 @@ def test_cli_reports_missing_gpu(tmp_path):
     r = subprocess.run([M.CLI_PATH, str(p)], capture_output=True, text=True)
     assert r.returncode != 0 and "GPU" in r.stderr
     assert "INPUT:: Simulation Parameters" in r.stdout
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU failure path")
+def test_equipotential_needs_the_gpu():
+    cfg = dict(PUBLISHED["c60_He"], Equipotential="yes")
+    with pytest.raises(M.MassCCSError) as e:
+        prepared(cfg)
+    assert e.value.code == 5
