@@ -81,5 +81,7 @@ int main(int argc, char **argv) {
   mccs_engine_destroy(eng);
   mccs_prepared_destroy(prep);
   std::cout << "Program finished..." << std::endl;
+  // the reference's System destructor runs after main's last line
+  if (cfg.equipotential) std::cout << "equipotential destroyed" << std::endl;
   return 0;
 }
