@@ -220,7 +220,8 @@ typedef struct {
   double skin;             /* Verlet-list skin in A; 0: tuned (or 1 A) */
 } mccs_batch_opts;
 
-/* The targets must outlive nothing: their data is copied to the device. */
+/* The targets' data is copied to the device: the mccs_prepared handles may be
+ * destroyed once the batch exists. */
 mccs_status mccs_batch_create(const mccs_prepared *const *targets, int32_t ntargets,
                               const mccs_batch_opts *opts, mccs_batch **out);
 void mccs_batch_destroy(mccs_batch *b);

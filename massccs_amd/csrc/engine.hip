@@ -600,6 +600,20 @@ __global__ void targets_kernel(const double *__restrict__ uN, const double *__re
 
 namespace {
 constexpr double SUR_DCAP = 12.0;  // distances are resolved up to this
+
+// Device buffer released on scope exit (error paths included).
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { HIP_OK(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1))); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  operator T *() const { return p; }
+};
 double secs_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -635,11 +649,11 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
   pts.reserve(3 * want);
   std::mt19937_64 rng(o.seed ? o.seed : 2104);
   std::uniform_real_distribution<double> U01(0.0, 1.0);
-  float *dp = nullptr, *dd = nullptr;
-  HIP_OK(hipMalloc(&dp, sizeof(float) * 3 * chunk));
-  HIP_OK(hipMalloc(&dd, sizeof(float) * chunk));
   std::vector<float> hp(3 * chunk), hd(chunk);
   std::vector<double> cand(3 * chunk);
+  DevBuf<double> dx(3 * want), fN(3 * want), uN(want), fD(3 * want), uD(want);
+  {
+  DevBuf<float> dp(3 * chunk), dd(chunk);
   for (int guard = 0; pts.size() < 3 * want; guard++) {
     if (guard > 1000) throw Error(MCCS_ERR_INVALID, "no room outside the shell to sample");
     for (uint64_t i = 0; i < chunk; i++)
@@ -657,15 +671,8 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
       if (hd[i] >= dmin && hd[i] < T.rlong)  // outside the shell, inside the cutoff
         for (int d = 0; d < 3; d++) pts.push_back((double)hp[3 * i + d]);  // the float point itself
   }
-  hipFree(dp);
-  hipFree(dd);
+  }
   // exact energies (and forces, for validation) of both site roles
-  double *dx = nullptr, *fN = nullptr, *uN = nullptr, *fD = nullptr, *uD = nullptr;
-  HIP_OK(hipMalloc(&dx, sizeof(double) * 3 * want));
-  HIP_OK(hipMalloc(&fN, sizeof(double) * 3 * want));
-  HIP_OK(hipMalloc(&uN, sizeof(double) * want));
-  HIP_OK(hipMalloc(&fD, sizeof(double) * 3 * want));
-  HIP_OK(hipMalloc(&uD, sizeof(double) * want));
   HIP_OK(hipMemcpyAsync(dx, pts.data(), sizeof(double) * 3 * want, hipMemcpyHostToDevice, st));
   eval_sites_dev(E, MCCS_SITE_N, dx, want, fN, uN);
   eval_sites_dev(E, MCCS_SITE_DUMMY, dx, want, fD, uD);
@@ -715,11 +722,7 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
   // 3. train on the first ntr points (device-resident FP32 positions and
   //    scaled energies)
   auto t1 = std::chrono::steady_clock::now();
-  float *txyz = nullptr, *te = nullptr, *tdir = nullptr, *tder = nullptr;
-  HIP_OK(hipMalloc(&txyz, sizeof(float) * 3 * ntr));
-  HIP_OK(hipMalloc(&te, sizeof(float) * 2 * ntr));
-  HIP_OK(hipMalloc(&tdir, sizeof(float) * 3 * ntr));
-  HIP_OK(hipMalloc(&tder, sizeof(float) * 2 * ntr));
+  DevBuf<float> txyz(3 * ntr), te(2 * ntr), tdir(3 * ntr), tder(2 * ntr);
   double wd = 1.0;
   {  // a random direction per point and the exact directional derivatives
      // -F . dir of both outputs (scaled), matched by central differences
@@ -764,15 +767,6 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
   to.fd_h = 0.2;
   to.w_deriv = wd;
   rep.train_loss = mlp_train(E->mlp, txyz, te, ntr, to, nullptr);
-  hipFree(txyz);
-  hipFree(te);
-  hipFree(tdir);
-  hipFree(tder);
-  hipFree(dx);
-  hipFree(fN);
-  hipFree(uN);
-  hipFree(fD);
-  hipFree(uD);
   rep.seconds_train = secs_since(t1);
 
   // 4. held-out error of energies and forces (F = -grad U)
@@ -812,18 +806,10 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
   E->d_sg = nullptr;
   HIP_OK(hipMalloc(&E->d_sg, sizeof(float4) * nodes));
   const uint64_t gch = 1u << 20;
-  float *gp = nullptr, *gd = nullptr, *gy = nullptr;
-  double *g64 = nullptr, *gf = nullptr, *gu0 = nullptr, *gu1 = nullptr;
-  HIP_OK(hipMalloc(&gp, sizeof(float) * 3 * gch));
-  HIP_OK(hipMalloc(&gd, sizeof(float) * gch));
-  HIP_OK(hipMalloc(&gy, sizeof(float) * 2 * gch));
+  DevBuf<float> gp(3 * gch), gd(gch), gy(2 * gch);
   const bool exact_src = o.table_source == 1;
-  if (exact_src) {
-    HIP_OK(hipMalloc(&g64, sizeof(double) * 3 * gch));
-    HIP_OK(hipMalloc(&gf, sizeof(double) * 3 * gch));
-    HIP_OK(hipMalloc(&gu0, sizeof(double) * gch));
-    HIP_OK(hipMalloc(&gu1, sizeof(double) * gch));
-  }
+  DevBuf<double> g64(exact_src ? 3 * gch : 1), gf(exact_src ? 3 * gch : 1),
+      gu0(exact_src ? gch : 1), gu1(exact_src ? gch : 1);
   // the network is tapered to zero over the last 4 A before the cutoff so
   // that, splines included, it vanishes where the exact field does
   const float tap0 = exact_src ? 1e30f : (float)(T.rlong - 5.0);
@@ -851,15 +837,6 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
     HIP_OK(hipGetLastError());
   }
   HIP_OK(hipStreamSynchronize(st));
-  hipFree(gp);
-  hipFree(gd);
-  hipFree(gy);
-  if (exact_src) {
-    hipFree(g64);
-    hipFree(gf);
-    hipFree(gu0);
-    hipFree(gu1);
-  }
   rep.seconds_table = secs_since(t2);
   rep.n_train = ntr;
   rep.grid_nodes = nodes;
