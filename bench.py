@@ -53,6 +53,30 @@ def workload_cfg(tmp):
             "dt": 10.0, "Temp": 298.0, "numberProbe": 10000, "nIter": 10}
 
 
+def small_cfg(tmp, which):
+    """Other single-target workloads: the C60 Lennard-Jones case the reference
+    runs on its CPU (He, force type 2, 3 x 10^4 trajectories) and the
+    published ubiquitin / He configuration (force type 4, 10 x 10^4)."""
+    from massccs_amd import synth
+    common = {"seed": 2104, "dt": 10.0, "Temp": 298.0}
+    if which == "c60":
+        xyz = os.path.join(tmp, "c60_analytic.xyz")
+        synth.write_xyz(xyz, ["C"] * 60, synth.c60(), comment="C60 truncated icosahedron")
+        ff = os.path.join(tmp, "c60_he.ff")
+        with open(ff, "w") as fh:
+            fh.write("1\nSymbol mass eps sigma\nC 12.011 0.041473 3.02\n")
+        return dict(common, targetFileName=xyz, **{"force-field": ff}, GasBuffer="He",
+                    **{"Short-range cutoff": "yes", "Long-range forces": "no"},
+                    polarizability="no", numberProbe=10000, nIter=3), \
+            "plain FP64, C60 (analytic, LJ only) + He, force type 2, 3x10^4 trajectories"
+    pqr = os.path.join(HERE, "tests", "golden", "04_1ubq_charge_4e_min.pqr")
+    return dict(common, targetFileName=pqr, GasBuffer="He", polarizability="yes",
+                **{"Short-range cutoff": "yes", "LJ-cutoff": 12.0, "Long-range forces": "yes",
+                   "Long-range cutoff": "yes", "Coul-cutoff": 25.0},
+                numberProbe=10000, nIter=10), \
+        "plain FP64, ubiquitin 04_1ubq_charge_4e_min.pqr + He, force type 4, 10x10^4 trajectories"
+
+
 def host_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -133,7 +157,7 @@ def run_big(args, which):
     cfg, (s, xyz, q), label = big_target_cfg(which)
     n = args.traj or (10 ** 6 if which == "shell1m" else 10 ** 5)
     P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
-    E = M.Engine(P, count_work=True)
+    E = M.Engine(P, count_work=True, lanes=args.lanes, slots=args.slots)
     u = torch.from_numpy(P.samples(n)).cuda()
     rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.Stream()
@@ -265,10 +289,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="lanes per trajectory: 0 auto, 1 (one lane per trajectory, the "
+                         "wavefront evaluating forces together), 16, 32 or 64")
+    ap.add_argument("--slots", type=int, default=0, help="trajectories per wavefront with --lanes 1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="globule",
-                    choices=["globule", "batch64", "ellipsoid50k", "shell1m"],
+                    choices=["globule", "batch64", "ellipsoid50k", "shell1m", "c60", "ubq_he"],
                     help="globule: the measured configuration (1,231 atoms, N2, 10^5 "
                          "trajectories); the others are the batch and large-target runs")
     ap.add_argument("--traj", type=int, default=0, help="trajectories (per target for batch64)")
@@ -291,12 +318,17 @@ def main():
     import massccs_amd as M
 
     tmp = tempfile.mkdtemp(prefix="mccs_bench_")
-    cfg = workload_cfg(tmp)
+    if args.workload == "globule":
+        cfg = workload_cfg(tmp)
+        label = ("plain FP64, 1231-atom globule + N2, LJ+Coulomb+induced dipole with cutoffs "
+                 "(force type 6), 10x10^4 trajectories")
+    else:
+        cfg, label = small_cfg(tmp, args.workload)
     P = M.Prepared(M.config_from_dict(cfg))
     info = P.info
     n_iter, n_traj = cfg["nIter"], cfg["numberProbe"]
     n = n_iter * n_traj
-    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True)
+    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True, slots=args.slots)
 
     u = torch.from_numpy(P.samples(n)).cuda()
     rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
@@ -340,16 +372,16 @@ def main():
     flops = _flops(r)
     achieved = flops / (kern_ms * 1e-3) / 1e12
     value = n * args.steps / wall
-    site_evals = 3 * int(r["steps"].sum())
+    nsite = 3 if cfg["GasBuffer"] in ("N2", "CO2") else 1
+    site_evals = nsite * int(r["steps"].sum())
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "trajectories/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded globule target and mt19937 samples)",
-        "config": {"workload": "plain FP64, 1231-atom globule + N2, LJ+Coulomb+induced dipole "
-                               "with cutoffs (force type 6), 10x10^4 trajectories",
-                   "n_atoms": info.natoms, "trajectories_per_step": n, "probe": "N2",
+        "config": {"workload": label,
+                   "n_atoms": info.natoms, "trajectories_per_step": n, "probe": cfg["GasBuffer"],
                    "force_type": info.force_type, "lanes_per_trajectory": args.lanes or "auto",
                    "parallelism": "1 GPU"},
         "force_evals_per_s": round(site_evals / (kern_ms * 1e-3), 1),
@@ -358,7 +390,8 @@ def main():
                              "executed": int(executed["steps"])},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": TRAFFIC_BYTES, "kernel": "traj_kernel<N2, type 6>",
+                     "traffic": TRAFFIC_BYTES if args.workload == "globule" else None,
+                     "kernel": "traj_kernel<%s, type %d>" % (cfg["GasBuffer"], info.force_type),
                      "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
     }
     if not args.no_cpu_baseline:

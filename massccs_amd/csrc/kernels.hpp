@@ -51,6 +51,17 @@ constexpr double PI = 3.14159265358979323846;
 constexpr double FAR = 1.0e30;                               // parks an inactive site
 constexpr int MAX_STEPS = 4000000;                           // per try
 
+// Launch shape of the group kernel: threads per block (the largest block the
+// launcher may use) and wavefronts per SIMD the register budget is sized for.
+#ifndef MCCS_TRAJ_BLOCK
+#define MCCS_TRAJ_BLOCK 256
+#endif
+#ifndef MCCS_TRAJ_WAVES
+#define MCCS_TRAJ_WAVES 2
+#endif
+constexpr int TRAJ_BLOCK = MCCS_TRAJ_BLOCK;
+constexpr int TRAJ_WAVES = MCCS_TRAJ_WAVES;
+
 enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4 };
 
 // Kernel argument: target, grid and run constants.
@@ -1147,7 +1158,7 @@ __device__ bool resolve(const Queue &Q, mccs_traj_record *rec, uint64_t j, doubl
 // every working group performs exactly one force evaluation: the initial
 // force of a try, or the force of one velocity-Verlet/RATTLE step.
 template <int GK, int FT, int L, bool LDS, bool COUNT, bool MT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void traj_kernel(
+__global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ_WAVES))) void traj_kernel(
     const Tgt *__restrict__ tg, const unsigned long long *__restrict__ toff, int ntg,
     const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec, Queue Q) {
   static_assert(!(MT && LDS), "targets staged in LDS need a launch of their own");
