@@ -248,6 +248,7 @@ struct Shm {
   void *list;             // the group's Verlet list (uint16 indices when atoms are staged, else int)
   int *incl;              // per-lane inclusive run ends of the group
   int *base;              // per-lane (run start - run offset) * 2 + first-shell flag
+  double *red;            // 64-lane groups: reduction rows (shares incl/base's storage)
   const double4 *atoms;   // LDS copies when staged
   const double2 *ljp, *ljc;
 };
@@ -274,6 +275,9 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
                                            const double (&qs)[S::NS], const double4 at,
                                            const double2 lp, const double2 lc, int allow_lj,
                                            Acc (&a)[S::NS], Work &w) {
+  // per-atom factors shared by the sites
+  const double qk = at.w * KC;       // Coulomb: q_j K (times the site charge below)
+  const double m3q = -3.0 * at.w;    // field gradient: -3 q_j
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
     if (!(S::lj(s) || S::coul(s) || S::dip(s))) continue;
@@ -302,30 +306,32 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
           if (COUNT) w.lj++;
         }
         if (in_c) {
-          const double r = r2 * rinv;
-          const double sm = 1.0 - r * r2 * T.irc3;
+          // shifted force r^-3 (1 - r^3/rc^3) = r^-3 - rc^-3 (src/Force.cpp:671-683 and
+          // the smoothed field of :393-426)
+          const double r3 = rinv * r2inv;
+          const double r3s = r3 - T.irc3;
           if (S::coul(s)) {
-            const double qq = qs[s] * at.w * KC;
+            const double qq = qs[s] * qk;
             a[s].U += qq * (rinv + T.cc1 + T.cc2 * r2);
-            const double fc = qq * rinv * r2inv * sm;
+            const double fc = qq * r3s;
             a[s].fx += fc * dx;
             a[s].fy += fc * dy;
             a[s].fz += fc * dz;
             if (COUNT) w.coul++;
           }
           if (S::dip(s)) {
-            const double r3 = rinv * r2inv;
-            const double A = at.w * (r3 - T.irc3);
-            const double B = -3.0 * at.w * r3 * r2inv;
+            const double A = at.w * r3s;
+            const double B = m3q * r3 * r2inv;
             a[s].Ex += dx * A;
             a[s].Ey += dy * A;
             a[s].Ez += dz * A;
-            a[s].Exx += A + B * dx * dx;
-            a[s].Eyy += A + B * dy * dy;
-            a[s].Ezz += A + B * dz * dz;
-            a[s].Exy += B * dx * dy;
-            a[s].Exz += B * dx * dz;
-            a[s].Eyz += B * dy * dz;
+            const double bx = B * dx, by = B * dy, bz = B * dz;
+            a[s].Exx += A + bx * dx;
+            a[s].Eyy += A + by * dy;
+            a[s].Ezz += A + bz * dz;
+            a[s].Exy += bx * dy;
+            a[s].Exz += bx * dz;
+            a[s].Eyz += by * dz;
             if (COUNT) w.dip++;
           }
         }
@@ -439,8 +445,67 @@ __device__ void walk_runs(const Tgt &T, const Shm &sh, int lane, const RunGen &g
   }
 }
 
+// Sums of 64 lanes through LDS, four values at a time: row i of the group's
+// scratch holds value i of every lane; lane 16r+q adds the four entries
+// [4q, 4q+4) of row r, the 16 lanes of a row combine their partial sums with
+// four DPP steps, and the totals are read from lanes 0, 16, 32, 48 into
+// scalar registers (every lane receives the same bits; the order of every
+// addition is fixed).
+constexpr int RROWS = 4, RSTRIDE = 66;  // rows per pass, doubles per row (64 + pad)
+template <int K>
+__device__ __forceinline__ void sum64(double *const (&v)[K], double *red, int lane) {
+#pragma unroll
+  for (int c0 = 0; c0 < K; c0 += RROWS) {
+    lds_fence();
+#pragma unroll
+    for (int i = 0; i < RROWS; i++)
+      if (c0 + i < K) red[i * RSTRIDE + lane] = *v[c0 + i];
+    lds_fence();
+    const int r = lane >> 4, q = lane & 15;
+    double x = 0.0;
+    if (c0 + r < K) {
+      const double2 *row = reinterpret_cast<const double2 *>(red + r * RSTRIDE + 4 * q);
+      const double2 a0 = row[0], a1 = row[1];
+      x = (a0.x + a0.y) + (a1.x + a1.y);
+    }
+    x += dpp<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dpp<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dpp<0x141>(x);  // row_half_mirror
+    x += dpp<0x140>(x);  // row_mirror: the 16 lanes of a row hold its sum
+#pragma unroll
+    for (int i = 0; i < RROWS; i++)
+      if (c0 + i < K) *v[c0 + i] = readlane_d(x, 16 * i);
+  }
+  lds_fence();
+}
+
 template <class S, int L>
-__device__ __forceinline__ void reduce(Acc (&a)[S::NS]) {
+__device__ __forceinline__ void reduce(Acc (&a)[S::NS], double *red) {
+  if constexpr (L == 64) {
+    constexpr int K = [] {
+      int k = 0;
+      for (int s = 0; s < S::NS; s++) {
+        if (S::lj(s) || S::coul(s) || S::dip(s)) k += 4;
+        if (S::dip(s)) k += 9;
+      }
+      return k;
+    }();
+    double *v[K];
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < S::NS; s++) {
+      if (S::lj(s) || S::coul(s) || S::dip(s)) {
+        v[k++] = &a[s].U; v[k++] = &a[s].fx; v[k++] = &a[s].fy; v[k++] = &a[s].fz;
+      }
+      if (S::dip(s)) {
+        v[k++] = &a[s].Ex; v[k++] = &a[s].Ey; v[k++] = &a[s].Ez;
+        v[k++] = &a[s].Exx; v[k++] = &a[s].Eyy; v[k++] = &a[s].Ezz;
+        v[k++] = &a[s].Exy; v[k++] = &a[s].Exz; v[k++] = &a[s].Eyz;
+      }
+    }
+    sum64<K>(v, red, threadIdx.x & 63);
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
     if (S::lj(s) || S::coul(s) || S::dip(s)) {
@@ -486,10 +551,13 @@ __device__ __forceinline__ void finish(const Tgt &T, const Acc (&a)[S::NS], int 
 // Verlet list of one group: every atom within R of the centre c0 it was built
 // around.  It serves while |c - c0| + off <= R - rlong, i.e. while every atom
 // that can be inside the longest cutoff of any site is on the list.
+// The list holds first the atoms within Rin (the LJ cutoff plus the same
+// margin) and then the others: only the first nin entries can be inside the
+// LJ cutoff of a site while the list serves.
 struct VList {
   double c0[3];
   double R;
-  int n;
+  int n, nin;
   int valid;  // -1 none, 0 overflowed its capacity (walk the columns instead), 1 usable
 };
 
@@ -498,7 +566,7 @@ using LIdx = typename std::conditional<LDS, uint16_t, int>::type;
 
 template <class S, int L, bool LDS>
 __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, const double c[3],
-                           double R) {
+                           double R, double Rin) {
   RunGen g;
   g.mode = 0;
   g.c[0] = c[0];
@@ -512,8 +580,12 @@ __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, con
   g.ix0 = ix0;
   g.iy0 = iy0;
   g.nys = iy1 - iy0 + 1;
-  const double R2 = R * R;
-  int n = 0;
+  const double R2 = R * R, Rin2 = Rin * Rin;
+  int n = 0, nin = 0;
+  // two passes over the same columns: atoms within Rin, then the rest within R
+  for (int pass = 0; pass < 2; pass++) {
+  const double lo2 = pass ? Rin2 : -1.0, hi2 = pass ? R2 : Rin2;
+  if (pass) nin = n;
   // every lane of the group runs the same number of visits per chunk of L
   // candidates (walk_runs strides t by L), so the ballot sees the whole group
   for (int r0 = 0; r0 < (ix1 - ix0 + 1) * g.nys; r0 += L) {
@@ -534,7 +606,8 @@ __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, con
         a = sh.base[cur] + t;
         const double4 at = LDS ? sh.atoms[a] : T.atoms[a];
         const double ex = at.x - c[0], ey = at.y - c[1], ez = at.z - c[2];
-        keep = ex * ex + ey * ey + ez * ez < R2;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        keep = d2 >= lo2 && d2 < hi2;
       }
       int below, cnt;
       group_ballot<L>(keep, lane, below, cnt);
@@ -543,6 +616,8 @@ __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, con
     }
     lds_fence();
   }
+  }
+  vl.nin = nin;
   vl.c0[0] = c[0];
   vl.c0[1] = c[1];
   vl.c0[2] = c[2];
@@ -575,7 +650,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
   if (!S::CUT) {
     g.mode = 2;
     walk_runs<L>(T, sh, lane, g, 1, visit);
-    reduce<S, L>(a);
+    reduce<S, L>(a, sh.red);
     finish<S>(T, a, -1, f, U);
     return;
   }
@@ -616,11 +691,11 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       const double moved = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
       if (vl->valid < 0 || moved + off + T.rlong + 1e-9 > vl->R) {
         VList nl;
-        build_list<S, L, LDS>(T, sh, lane, nl, c, T.rlong + off + T.skin);
+        build_list<S, L, LDS>(T, sh, lane, nl, c, T.rlong + off + T.skin, T.rlj + off + T.skin);
         if (lane == 0) *vl = nl;
         lds_fence();
       }
-      const int n = vl->n;
+      const int n = vl->n, nin = vl->nin;
       if (vl->valid > 0) {
         const LIdx<LDS> *lst = static_cast<const LIdx<LDS> *>(sh.list);
         // software-pipelined: the next atom's data is read while this one's
@@ -632,9 +707,9 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
           const double4 cat = at;
           const double2 clp = lp, clc = lc;
           if (t + L < n) fetch<LDS, CEN>(T, sh, (int)lst[t + L], at, lp, lc);
-          pair_terms<S, COUNT>(T, p, qs, cat, clp, clc, 1, a, w);
+          pair_terms<S, COUNT>(T, p, qs, cat, clp, clc, t < nin, a, w);
         }
-        reduce<S, L>(a);
+        reduce<S, L>(a, sh.red);
         finish<S>(T, a, -1, f, U);
         return;
       }
@@ -654,7 +729,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     g.iy0 = iy0;
     g.nys = iy1 - iy0 + 1;
     walk_runs<L>(T, sh, lane, g, (ix1 - ix0 + 1) * g.nys, visit);
-    reduce<S, L>(a);
+    reduce<S, L>(a, sh.red);
     finish<S>(T, a, -1, f, U);
     return;
   }
@@ -689,7 +764,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       pair_terms<S, COUNT>(T, pp, qs, at, lp, lc, first, b, w);
     };
     walk_runs<L>(T, sh, lane, g, nr, visit1);
-    reduce<S, L>(b);
+    reduce<S, L>(b, sh.red);
     finish<S>(T, b, ps, f, U);
   }
 }
@@ -987,15 +1062,21 @@ __device__ void spawn_rest(const Queue &Q, uint64_t j) {
                          __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Block LDS layout: atoms and LJ pairs (when staged) | per-lane run scratch
-// (2 ints per thread) | one GState per group | one Verlet list per group.
+// Per-group scratch: the run scratch of the column walks (2 ints per lane),
+// which 64-lane groups share with their reduction rows.
+__host__ __device__ constexpr size_t group_scratch(int lanes) {
+  return lanes == 64 ? sizeof(double) * RROWS * RSTRIDE : sizeof(int) * 2 * (size_t)lanes;
+}
+
+// Block LDS layout: atoms and LJ pairs (when staged) | per-group scratch |
+// one GState per group | one Verlet list per group.
 __host__ __device__ inline size_t lds_bytes(int natoms, int list_cap, int block, int lanes,
                                             bool lds_atoms, bool central) {
   const size_t lidx = lds_atoms ? sizeof(uint16_t) : sizeof(int);
   size_t b = 0;
   if (lds_atoms)
     b += (sizeof(double4) + sizeof(double2) * (central ? 2 : 1)) * (size_t)natoms;
-  b += sizeof(int) * 2 * (size_t)block;
+  b += group_scratch(lanes) * (size_t)(block / lanes);
   b += sizeof(GState) * (size_t)(block / lanes);
   b += lidx * (size_t)list_cap * (size_t)(block / lanes);
   return b + 64;
@@ -1010,8 +1091,8 @@ __device__ void stage(const Tgt &T, Shm &sh, char *smem) {
   if (LDS) p += sizeof(double2) * (size_t)T.natoms;
   double2 *ljc = reinterpret_cast<double2 *>(p);
   if (LDS && CEN) p += sizeof(double2) * (size_t)T.natoms;
-  int *scratch = reinterpret_cast<int *>(p);
-  p += sizeof(int) * 2 * blockDim.x;
+  char *scratch = p;
+  p += group_scratch(L) * (blockDim.x / L);
   GState *states = reinterpret_cast<GState *>(p);
   p += sizeof(GState) * (blockDim.x / L);
   char *lists = p;
@@ -1027,8 +1108,9 @@ __device__ void stage(const Tgt &T, Shm &sh, char *smem) {
   sh.atoms = atoms;
   sh.ljp = ljp;
   sh.ljc = ljc;
-  sh.incl = scratch + 2 * L * g;
-  sh.base = scratch + 2 * L * g + L;
+  sh.incl = reinterpret_cast<int *>(scratch + group_scratch(L) * g);
+  sh.base = sh.incl + L;
+  sh.red = reinterpret_cast<double *>(sh.incl);
   sh.state = states + g;
   sh.list = lists + sizeof(LIdx<LDS>) * (size_t)T.list_cap * g;
 }

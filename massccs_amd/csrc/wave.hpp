@@ -176,10 +176,11 @@ __device__ __forceinline__ void pair_w(const Tgt &T, const double (&p)[S::NS][3]
 
 // Verlet list of one slot, built by the whole wavefront into the slot's HBM
 // list: every atom within R of c, in the order of the cell columns (the same
-// criterion and order as build_list).  Returns the count (> cap: overflowed).
+// criterion and order as build_list: atoms within Rin first).  Returns the
+// count (> cap: overflowed); nin receives the count within Rin.
 template <class S, bool LDS>
 __device__ int wbuild_list(const Tgt &T, const Shm &sh, int lane, LIdx<LDS> *__restrict__ out,
-                           int cap, const double c[3], double R) {
+                           int cap, const double c[3], double R, double Rin, int &nin) {
   RunGen g;
   g.mode = 0;
   g.c[0] = c[0];
@@ -193,9 +194,13 @@ __device__ int wbuild_list(const Tgt &T, const Shm &sh, int lane, LIdx<LDS> *__r
   g.ix0 = ix0;
   g.iy0 = iy0;
   g.nys = iy1 - iy0 + 1;
-  const double R2 = R * R;
+  const double R2 = R * R, Rin2 = Rin * Rin;
   const int nruns = (ix1 - ix0 + 1) * g.nys;
   int n = 0;
+  nin = 0;
+  for (int pass = 0; pass < 2; pass++) {
+  const double lo2 = pass ? Rin2 : -1.0, hi2 = pass ? R2 : Rin2;
+  if (pass) nin = n;
   for (int r0 = 0; r0 < nruns; r0 += 64) {
     int start = 0, len = 0, first = 1;
     if (r0 + lane < nruns) g.get(T, r0 + lane, start, len, first);
@@ -214,7 +219,8 @@ __device__ int wbuild_list(const Tgt &T, const Shm &sh, int lane, LIdx<LDS> *__r
         a = sh.base[cur] + t;
         const double4 at = LDS ? sh.atoms[a] : T.atoms[a];
         const double ex = at.x - c[0], ey = at.y - c[1], ez = at.z - c[2];
-        keep = ex * ex + ey * ey + ez * ez < R2;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        keep = d2 >= lo2 && d2 < hi2;
       }
       const unsigned long long m = __ballot(keep);
       const int below = __popcll(m & ((1ull << lane) - 1ull));
@@ -222,6 +228,7 @@ __device__ int wbuild_list(const Tgt &T, const Shm &sh, int lane, LIdx<LDS> *__r
       n += __popcll(m);
     }
     lds_fence();
+  }
   }
   glist_fence();
   return n;
@@ -306,7 +313,9 @@ __device__ void wave_force(const Tgt &T, const Shm &sh, double *red, int lane,
       const double c[3] = {p[cs][0], p[cs][1], p[cs][2]};
       if (build) {
         const double R = T.rlong + off + T.skin;
-        const int n = wbuild_list<S, LDS>(T, sh, lane, lst, cap, c, R);
+        int nin = 0;
+        const int n = wbuild_list<S, LDS>(T, sh, lane, lst, cap, c, R, T.rlj + off + T.skin, nin);
+        vl.nin = nin;
         vl.c0[0] = c[0];
         vl.c0[1] = c[1];
         vl.c0[2] = c[2];
@@ -315,7 +324,7 @@ __device__ void wave_force(const Tgt &T, const Shm &sh, double *red, int lane,
         vl.valid = n <= cap;
       }
       if (vl.valid > 0) {
-        const int n = vl.n;
+        const int n = vl.n, nin = vl.nin;
         // two-stage pipeline: the index one chunk ahead is loaded from HBM
         // while the atom of the next chunk is read from LDS (or L2)
         double4 at = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -328,7 +337,7 @@ __device__ void wave_force(const Tgt &T, const Shm &sh, double *red, int lane,
           const double2 clp = lp, clc = lc;
           if (t + 64 < n) fetch<LDS, CEN>(T, sh, nxt, at, lp, lc);
           if (t + 128 < n) nxt = (int)lst[t + 128];
-          pair_w<S, COUNT>(T, p, qs, cat, clp, clc, 1, a, w);
+          pair_w<S, COUNT>(T, p, qs, cat, clp, clc, t < nin, a, w);
         }
       } else {
         // the list overflowed its capacity: walk the columns around the centre
@@ -424,6 +433,7 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(2))) voi
   sh.ljc = ljc;
   sh.incl = scratch;
   sh.base = scratch + 64;
+  sh.red = nullptr;
   using IDX = LIdx<LDS>;
   IDX *lists = static_cast<IDX *>(WL.base) +
                ((uint64_t)blockIdx.x * (WBLOCK / 64) + wave) * (uint64_t)WL.slots * WL.cap;
@@ -441,6 +451,7 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(2))) voi
   VList vl;
   vl.valid = -1;
   vl.n = 0;
+  vl.nin = 0;
   vl.R = 0.0;
   vl.c0[0] = vl.c0[1] = vl.c0[2] = 0.0;
   for (int q = 0; q < NP; q++)
@@ -674,6 +685,7 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(2))) voi
         lt.c0[2] = readlane_d(vl.c0[2], t);
         lt.R = readlane_d(vl.R, t);
         lt.n = readlane_i(vl.n, t);
+        lt.nin = readlane_i(vl.nin, t);
         lt.valid = readlane_i(vl.valid, t);
         double f[NS][3], Us[NS];
         WorkU wu{0, 0, 0, 0};
@@ -685,6 +697,7 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(2))) voi
           vl.c0[2] = (me ? lt.c0[2] : vl.c0[2]);
           vl.R = (me ? lt.R : vl.R);
           vl.n = (int)(me ? ((uint32_t)lt.n) : (uint32_t)vl.n);
+          vl.nin = (int)(me ? ((uint32_t)lt.nin) : (uint32_t)vl.nin);
           vl.valid = (int)(me ? ((uint32_t)lt.valid) : (uint32_t)vl.valid);
         }
         double Usum = 0.0;
