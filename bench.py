@@ -157,7 +157,7 @@ def run_big(args, which):
     cfg, (s, xyz, q), label = big_target_cfg(which)
     n = args.traj or (10 ** 6 if which == "shell1m" else 10 ** 5)
     P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
-    E = M.Engine(P, count_work=True, lanes=args.lanes, slots=args.slots)
+    E = M.Engine(P, count_work=True, lanes=args.lanes)
     u = torch.from_numpy(P.samples(n)).cuda()
     rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.Stream()
@@ -289,10 +289,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lanes", type=int, default=0,
-                    help="lanes per trajectory: 0 auto, 1 (one lane per trajectory, the "
-                         "wavefront evaluating forces together), 16, 32 or 64")
-    ap.add_argument("--slots", type=int, default=0, help="trajectories per wavefront with --lanes 1")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per trajectory: 0 auto, 16, 32 or 64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="globule",
                     choices=["globule", "batch64", "ellipsoid50k", "shell1m", "c60", "ubq_he"],
@@ -328,7 +325,7 @@ def main():
     info = P.info
     n_iter, n_traj = cfg["nIter"], cfg["numberProbe"]
     n = n_iter * n_traj
-    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True, slots=args.slots)
+    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True)
 
     u = torch.from_numpy(P.samples(n)).cuda()
     rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")

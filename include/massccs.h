@@ -126,9 +126,7 @@ typedef struct mccs_engine mccs_engine;
 
 typedef struct {
   int32_t device;          /* HIP device ordinal */
-  int32_t lanes;           /* lanes cooperating on one trajectory: 0 = auto, 16, 32 or 64;
-                              1 = one lane integrates each trajectory and the whole
-                              wavefront evaluates the site forces of its trajectories */
+  int32_t lanes;           /* lanes cooperating on one trajectory: 0 = auto, 16, 32 or 64 */
   int32_t block;           /* threads per block: 0 = auto, else 128 or 256 */
   int32_t use_lds;         /* -1 auto, 0 global-memory atoms, 1 atoms staged in LDS */
   int32_t count_work;      /* 1: fill the work counters of mccs_traj_record */
@@ -136,8 +134,6 @@ typedef struct {
                               fewer than this many trajectories are unresolved and no new
                               one is left, idle groups also start its later tries side by
                               side (results are the same either way): -1 = auto, 0 = never */
-  int32_t slots;           /* with lanes = 1: trajectories per wavefront, 0 = auto, 16, 32
-                              or 64 */
 } mccs_engine_opts;
 
 mccs_status mccs_engine_create(const mccs_prepared *p, const mccs_engine_opts *opts,

@@ -57,8 +57,7 @@ class PreparedInfo(C.Structure):
 
 class EngineOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("block", C.c_int32),
-                ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32),
-                ("slots", C.c_int32)]
+                ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32)]
 
 
 class BatchOpts(C.Structure):
@@ -260,8 +259,8 @@ class Engine:
     """Device engine on one GPU (target resident in HBM)."""
 
     def __init__(self, prepared, device=0, lanes=0, block=0, use_lds=-1, count_work=True,
-                 spec_below=-1, slots=0):
-        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below, slots)
+                 spec_below=-1):
+        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below)
         h = C.c_void_p()
         _check(lib().mccs_engine_create(prepared.h, C.byref(o), C.byref(h)))
         self.h, self.prepared = h, prepared

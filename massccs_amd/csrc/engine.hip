@@ -2,7 +2,6 @@
 // the host-side launcher behind the C ABI.
 #include "kernels.hpp"
 #include "mlp.hpp"
-#include "wave.hpp"
 
 #include <chrono>
 #include <deque>
@@ -80,15 +79,6 @@ __global__ void tally_final_kernel(const double *__restrict__ omega, uint64_t n_
 
 // ---------------------------------------------------------------- dispatch
 using EvalFn = void (*)(Tgt, const double *, uint64_t, double, double *, double *);
-
-WaveFn select_wave(int gk, int ft, bool lds, bool count) {
-  switch (gk) {
-    case 1: return select_wave_gk1(ft, lds, count);
-    case 2: return select_wave_gk2(ft, lds, count);
-    case 3: return select_wave_gk3(ft, lds, count);
-  }
-  return nullptr;
-}
 
 TrajFn select_traj(int gk, int ft, int L, bool lds, bool count) {
   switch (gk) {
@@ -224,13 +214,6 @@ struct Engine {
   int spec_below = -1;
   hipStream_t last_stream = nullptr;
   int nsites = 1;
-  // lane-per-trajectory path (lanes = 1)
-  bool wave = false, wave_lds = false;
-  int slots = 64, wave_cap = 0;
-  int period = 4, urgent = 4096;  // iteration period of ordinary tries; steps after which a first try is urgent
-  size_t wave_shmem = 0;
-  void *d_lists = nullptr;  // per-slot Verlet lists, grown on demand
-  size_t lists_bytes = 0;
   void *dalloc(size_t bytes) {
     void *p = nullptr;
     HIP_OK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
@@ -248,7 +231,6 @@ struct Engine {
     if (device >= 0) hipSetDevice(device);
     if (mlp) mlp_destroy(mlp);
     if (d_sg) hipFree(d_sg);
-    if (d_lists) hipFree(d_lists);
     for (void *p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
   }
@@ -307,6 +289,7 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   T.cx = P.corner[0]; T.cy = P.corner[1]; T.cz = P.corner[2];
   T.hx = P.half_box[0]; T.hy = P.half_box[1]; T.hz = P.half_box[2];
   T.ea = P.a; T.eb = P.b; T.ec = P.c; T.bmax = P.bmax;
+  T.iea = 1.0 / P.a; T.ieb = 1.0 / P.b; T.iec = 1.0 / P.c;
   const double rlj = P.cfg.lj_cutoff, rc = P.cfg.coul_cutoff;
   T.rlj = rlj; T.rc = rc;
   T.rlj2lo = rlj * rlj * (1.0 - 1e-15); T.rlj2hi = rlj * rlj * (1.0 + 1e-15);
@@ -344,34 +327,10 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   E->q.init(E->device);
 
   // launch shape: lanes per trajectory by target size unless given
-  int L = o.lanes;
-  if (L == 0) L = P.natoms <= 200 ? 16 : 64;
-  if (L != 1 && L != 16 && L != 32 && L != 64)
-    throw Error(MCCS_ERR_INVALID, "lanes must be 0, 1, 16, 32 or 64");
-  if (L == 1) {
-    // one lane per trajectory; the group path (64 lanes) remains for the
-    // surrogate and the per-point evaluator
-    E->wave = true;
-    L = 64;
-    const int sl = o.slots ? o.slots : 64;
-    if (sl < 1 || sl > 64 || (sl & (sl - 1)) != 0)
-      throw Error(MCCS_ERR_INVALID, "slots must be 0 or a power of two up to 64");
-    E->slots = sl;
-    if (const char *e = std::getenv("MASSCCS_WAVE_PERIOD")) E->period = std::max(1, std::atoi(e));
-    if (const char *e = std::getenv("MASSCCS_WAVE_URGENT")) E->urgent = std::max(1, std::atoi(e));
-    const bool cen = E->gk == 3;
-    // atoms staged in LDS when they fit next to the per-wavefront scratch
-    // (one block of 512 threads per CU); Verlet lists then hold 16-bit indices
-    const bool wfits = T.natoms <= 65535 && wave_lds_bytes(T.natoms, true, cen) <= 150 * 1024;
-    E->wave_lds = o.use_lds < 0 ? wfits : (o.use_lds != 0);
-    if (E->wave_lds && !wfits) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
-    E->wave_shmem = wave_lds_bytes(T.natoms, E->wave_lds, cen);
-    // lists: every atom when staged, else at most 16384 entries per slot (an
-    // overflowing list falls back to walking the cell columns)
-    E->wave_cap = E->wave_lds ? T.natoms : std::min(T.natoms, 16384);
-  } else if (o.slots != 0) {
-    throw Error(MCCS_ERR_INVALID, "slots applies to lanes = 1 only");
-  }
+  // launch shape: 64 lanes per trajectory unless given (measured fastest from
+  // 60 to 10^6 atoms: the wavefront's 64 lanes share each force evaluation)
+  int L = o.lanes ? o.lanes : 64;
+  if (L != 16 && L != 32 && L != 64) throw Error(MCCS_ERR_INVALID, "lanes must be 0, 16, 32 or 64");
   E->lanes = L;
   E->block = o.block ? o.block : TRAJ_BLOCK;
   if (E->block % 64 != 0 || E->block < 128 || E->block > TRAJ_BLOCK)
@@ -414,50 +373,11 @@ static void set_shmem(const void *fn, size_t bytes) {
     HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-// Lane-per-trajectory launch: one block of 512 threads per CU (as occupancy
-// allows), `slots` trajectories per wavefront, one Verlet list per slot.
-static void run_wave(Engine *E, const double *u_dev, uint64_t n, mccs_traj_record *rec_dev,
-                     hipStream_t s) {
-  WaveFn fn = select_wave(E->gk, E->ft, E->wave_lds, E->count);
-  if (!fn) {  // force types without a counter-free variant count their work
-    E->count = true;
-    fn = select_wave(E->gk, E->ft, E->wave_lds, true);
-  }
-  if (!fn) throw Error(MCCS_ERR_UNSUPPORTED, "no trajectory kernel for this gas / force type");
-  set_shmem((const void *)fn, E->wave_shmem);
-  int per_cu = 0;
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, WBLOCK, E->wave_shmem));
-  if (per_cu < 1) throw Error(MCCS_ERR_DEVICE, "trajectory kernel does not fit on a CU");
-  constexpr uint64_t waves_per_block = WBLOCK / 64;
-  uint64_t blocks = (uint64_t)per_cu * E->cus;
-  blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, (n + waves_per_block - 1) / waves_per_block));
-  const uint64_t nslots = blocks * waves_per_block * (uint64_t)E->slots;
-  const size_t need = nslots * (size_t)E->wave_cap * (E->wave_lds ? sizeof(uint16_t) : sizeof(int));
-  if (need > E->lists_bytes) {
-    if (E->d_lists) HIP_OK(hipFree(E->d_lists));
-    E->d_lists = nullptr;
-    E->lists_bytes = 0;
-    HIP_OK(hipMalloc(&E->d_lists, std::max<size_t>(need, 256)));
-    E->lists_bytes = need;
-  }
-  Queue Q = E->q.make(n, E->spec_below >= 0 ? E->spec_below : (long long)nslots);
-  launch_queue_init(Q, n, s);
-  const Tgt *tg = E->d_tg;
-  WList WL{E->d_lists, E->wave_cap, E->slots, E->period, E->urgent};
-  void *args[] = {&tg, &u_dev, &n, &rec_dev, &Q, &WL};
-  HIP_OK(hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(WBLOCK), args, E->wave_shmem, s));
-  E->last_stream = s;
-}
-
 void engine_run_device(Engine *E, const double *u_dev, uint64_t n, mccs_traj_record *rec_dev,
                        void *stream) {
   if (n == 0) return;
   HIP_OK(hipSetDevice(E->device));
   hipStream_t s = stream ? (hipStream_t)stream : E->stream;
-  if (E->wave && E->T.sg == nullptr) {  // the surrogate runs on the group path
-    run_wave(E, u_dev, n, rec_dev, s);
-    return;
-  }
   TrajFn fn = select_traj(E->gk, E->ft, E->lanes, E->lds, E->count);
   if (!fn) {  // force types without tuned variants run 64 lanes with work counters
     E->lanes = 64;

@@ -77,6 +77,7 @@ struct Tgt {
   double cell, cx, cy, cz;  // cell edge and grid corner
   double hx, hy, hz;        // half box
   double ea, eb, ec, bmax;  // ellipsoid semi-axes, maximal impact parameter
+  double iea, ieb, iec;     // their reciprocals (a pre-test of the exit condition only)
   double rlj, rc, rlj2lo, rlj2hi, rc2lo, rc2hi;
   double irc3, cc1, cc2;    // 1/rc^3, -1.5/rc, 0.5/rc^3
   double rc6inv;            // 1/rlj^6 (LJ energy shift)
@@ -687,9 +688,10 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     const double off = sqrt(off2);
     const double c[3] = {p[cs][0], p[cs][1], p[cs][2]};
     if (vl) {
+      // rebuild when moved + off + rlong + 1e-9 > R, compared in squares
       const double e0 = c[0] - vl->c0[0], e1 = c[1] - vl->c0[1], e2 = c[2] - vl->c0[2];
-      const double moved = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-      if (vl->valid < 0 || moved + off + T.rlong + 1e-9 > vl->R) {
+      const double room = vl->R - (off + T.rlong + 1e-9);
+      if (vl->valid < 0 || room < 0.0 || e0 * e0 + e1 * e1 + e2 * e2 > room * room) {
         VList nl;
         build_list<S, L, LDS>(T, sh, lane, nl, c, T.rlong + off + T.skin, T.rlj + off + T.skin);
         if (lane == 0) *vl = nl;
@@ -779,6 +781,27 @@ __device__ __forceinline__ double anglevec(const double *a, const double *b) {
   const double arg = dot3(a, b) / (sqrt(dot3(a, a)) * sqrt(dot3(b, b)));
   if (fabs(arg) > 1.0) return 0.0;
   return acos(arg);
+}
+// The same with |b| given (the previous step's |a|, the same bits) and |a|
+// returned for the next step.
+__device__ __forceinline__ double anglevec_nb(const double *a, const double *b, double nb,
+                                              double &na) {
+  na = sqrt(dot3(a, a));
+  const double arg = dot3(a, b) / (na * nb);
+  if (fabs(arg) > 1.0) return 0.0;
+  return acos(arg);
+}
+
+// The reference's exit test (rx/a)^2 + (ry/b)^2 + (rz/c)^2 > 1
+// (src/System.cpp:482-486), decided with reciprocals when the sum is not
+// within 1e-12 of 1 (the quotients differ from the products by a few ulp).
+__device__ __forceinline__ bool outside_ellipsoid(const Tgt &T, const double *r) {
+  const double x = r[0] * T.iea, y = r[1] * T.ieb, z = r[2] * T.iec;
+  const double q = x * x + y * y + z * z;
+  if (q < 1.0 - 1e-12) return false;
+  if (q > 1.0 + 1e-12) return true;
+  return (r[0] / T.ea) * (r[0] / T.ea) + (r[1] / T.eb) * (r[1] / T.eb) + (r[2] / T.ec) * (r[2] / T.ec) >
+         1.0;
 }
 
 __device__ __forceinline__ double kinetic(double m, const double *v) {
@@ -979,6 +1002,7 @@ struct GState {
   double pos[3][3], vel[3][3], fs[3][3];
   double rcm[3], vcm[3], rold[3], vcm_i[3];
   double Ei, dt, dtheta, hdm, Utot;
+  double rnorm;                 // |previous centre| (the angle sum's |b|)
   Rat rc;
   VList vl;
   unsigned long long j;         // trajectory (index over all targets of the launch)
@@ -1452,10 +1476,11 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
             st.vcm[d] = vcm[d];
           }
       }
-      double rold[3];
+      double rold[3], na;
       for (int d = 0; d < 3; d++) rold[d] = st.rold[d];
-      const double dth = st.dtheta + anglevec(rcm, rold);
+      const double dth = st.dtheta + anglevec_nb(rcm, rold, st.rnorm, na);
       if (lane == 0) {
+        st.rnorm = na;
         st.dtheta = dth;
         for (int d = 0; d < 3; d++) {
           st.rold[d] = rcm[d];
@@ -1546,6 +1571,7 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         if (GK == 3) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
         st.dtheta = 0.0;
         for (int d = 0; d < 3; d++) st.rold[d] = st.rcm[d];
+        st.rnorm = sqrt(dot3(st.rcm, st.rcm));
       }
       status = ST_STEP;
       lds_fence();
@@ -1597,9 +1623,7 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
     } else if (st.dtheta > 1.5 * PI) {
       code = TRY_ANGLE;
     } else if (fabs(rcm[0]) >= T.hx || fabs(rcm[1]) >= T.hy || fabs(rcm[2]) >= T.hz ||
-               (rcm[0] / T.ea) * (rcm[0] / T.ea) + (rcm[1] / T.eb) * (rcm[1] / T.eb) +
-                       (rcm[2] / T.ec) * (rcm[2] / T.ec) >
-                   1.0) {
+               outside_ellipsoid(T, rcm)) {
       const double Ei = st.Ei, Ut = st.Utot;
       const double dE = S::NS == 1 ? fabs(Ut + Ek - Ei) / Ei * 100.0
                                    : fabs((Ut + Ek - Ei) / Ei) * 100.0;

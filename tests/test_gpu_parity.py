@@ -239,83 +239,3 @@ def test_restarts_run_side_by_side_with_identical_results(workdir):
     assert np.array_equal(ref["tries"], r["tries"][idx])
     assert np.array_equal(ref["success"], r["success"][idx])
     assert np.all(np.abs(ref["steps"] - r["steps"][idx].astype(np.int64)) <= 0.02 * ref["steps"])
-
-
-# ---------------------------------------------------------------- one lane per trajectory
-WAVE_CASES = [
-    ("c60_He_t4", dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", **LONG), 2000, 0.999, 0.99),
-    ("ubq_He_t4", dict(targetFileName=UBQ_PQR, GasBuffer="He", polarizability="yes", **LONG), 400, 0.99, 0.97),
-    ("ubq_N2_t6", dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="yes", **LONG), 200, 0.98, 0.95),
-    ("c60_Ar_t4", dict(targetFileName=C60_XYZ, GasBuffer="Ar", polarizability="yes", **LONG), 1000, 0.98, 0.95),
-    ("ubq_CO2_t6", dict(targetFileName=UBQ_PQR, GasBuffer="CO2", polarizability="yes", **LONG), 200, 0.98, 0.95),
-    ("c60_co2_t4", dict(targetFileName=C60_XYZ, GasBuffer="co2", polarizability="yes", **LONG), 600, 0.98, 0.95),
-    ("ubq_N2_t3", dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="no",
-                       **{"Short-range cutoff": "no", "Long-range forces": "yes",
-                          "Long-range cutoff": "no"}), 100, 0.98, 0.95),
-]
-
-
-@pytest.mark.parametrize("name,cfg,n,flow_min,chi_min", WAVE_CASES, ids=[c[0] for c in WAVE_CASES])
-def test_wave_trajectories_match_oracle(name, cfg, n, flow_min, chi_min):
-    """One lane per trajectory, the wavefront evaluating the forces of its
-    slots: same samples, same physics as the oracle."""
-    cfg = dict(cfg, seed=2104)
-    print("case wave", name, flush=True)
-    P = prepared(cfg)
-    orc = oracle_for(P, cfg)
-    u = P.samples(n)
-    ref = orc.run(u)
-    rec = M.Engine(P, lanes=1).run(u)
-    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "wave_" + name)
-    assert flow >= flow_min
-    assert ok_big.mean() >= chi_min and ok_small.all()
-
-
-def test_wave_c60_lj(workdir):
-    cfg = c60_lj_cfg(workdir)
-    print("case wave c60_LJ", flush=True)
-    P = prepared(cfg)
-    orc = oracle_for(P, cfg)
-    u = P.samples(3000)
-    ref = orc.run(u)
-    rec = M.Engine(P, lanes=1).run(u)
-    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "wave_c60_LJ")
-    assert flow >= 0.999
-    assert ok_big.mean() >= 0.99 and ok_small.all()
-
-
-def test_wave_globule_lds_and_global_atoms(workdir):
-    """The globule with its atoms staged in LDS (16-bit lists) and read from
-    HBM (32-bit lists): both match the oracle and each other bit for bit."""
-    cfg = globule_cfg(workdir)
-    print("case wave globule_N2", flush=True)
-    P = prepared(cfg)
-    orc = oracle_for(P, cfg)
-    u = P.samples(300)
-    ref = orc.run(u)
-    rec = M.Engine(P, lanes=1, use_lds=1).run(u)
-    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "wave_globule_N2")
-    assert flow >= 0.98
-    assert ok_big.mean() >= 0.95 and ok_small.all()
-    glob = M.Engine(P, lanes=1, use_lds=0).run(u)
-    assert np.array_equal(glob.view(np.uint8), rec.view(np.uint8))
-
-
-def test_wave_records_do_not_depend_on_the_schedule(workdir):
-    """Slots per wavefront and side-by-side tries change which lane runs which
-    try and when; the records stay byte-identical (per-try lists, fixed
-    reduction order)."""
-    cfg = globule_cfg(workdir)
-    print("case wave schedule", flush=True)
-    P = prepared(cfg)
-    u = P.samples(4000)
-    runs = []
-    for slots, spec in ((64, -1), (16, 0), (32, 10 ** 9)):
-        E = M.Engine(P, lanes=1, slots=slots, spec_below=spec)
-        runs.append(E.run(u))
-        w = E.last_work()
-        assert w["steps"] >= runs[-1]["steps"].sum()
-        E.close()
-    for r in runs[1:]:
-        assert np.array_equal(r.view(np.uint8), runs[0].view(np.uint8))
-    assert (runs[0]["tries"] >= 4).sum() >= 5
