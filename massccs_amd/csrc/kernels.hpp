@@ -187,6 +187,16 @@ __device__ __forceinline__ double gsum(double v) {
 }
 
 template <int L>
+__device__ __forceinline__ uint32_t gsum_u(uint32_t v);
+
+// A work counter's total over the group (see count_of).
+template <int L>
+__device__ __forceinline__ uint32_t wtotal(uint32_t v) {
+  if constexpr (L == 64) return (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+  else return gsum_u<L>(v);
+}
+
+template <int L>
 __device__ __forceinline__ uint32_t gsum_u(uint32_t v) {
 #pragma unroll
   for (int d = 1; d < L; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, L);
@@ -271,7 +281,16 @@ __device__ __forceinline__ void fetch(const Tgt &T, const Shm &sh, int a, double
 // Pair terms of one candidate atom against every site.  allow_lj is 0 for
 // second-shell cells of an out-of-grid site (the reference applies LJ on the
 // first shell only).
-template <class S, bool COUNT>
+// Work counters: per lane (summed over the group at the end of a try), or,
+// for whole-wavefront groups (UNI), the wavefront's count from a ballot,
+// accumulated uniformly in scalar registers (lane 0 takes part in every
+// iteration of the strided loops, so its count is the total).
+template <bool UNI>
+__device__ __forceinline__ uint32_t count_of(bool c) {
+  return UNI ? (uint32_t)__popcll(__ballot(c)) : (uint32_t)c;
+}
+
+template <class S, bool COUNT, bool UNI = false>
 __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS][3],
                                            const double (&qs)[S::NS], const double4 at,
                                            const double2 lp, const double2 lc, int allow_lj,
@@ -284,7 +303,7 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
     if (!(S::lj(s) || S::coul(s) || S::dip(s))) continue;
     const double dx = p[s][0] - at.x, dy = p[s][1] - at.y, dz = p[s][2] - at.z;
     const double r2 = dx * dx + dy * dy + dz * dz;
-    if (COUNT) w.visits++;
+    if (COUNT) w.visits += count_of<UNI>(true);
     const double lj1 = S::cen(s) ? lc.x : lp.x, lj2 = S::cen(s) ? lc.y : lp.y;
     if (S::CUT) {
       // r < cutoff as the reference tests it (sqrt(r2) < rc), decided on r2
@@ -293,6 +312,11 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
       if (S::lj(s))
         in_lj = allow_lj && (r2 < T.rlj2lo || (r2 < T.rlj2hi && sqrt(r2) < T.rlj));
       if (S::coul(s) || S::dip(s)) in_c = (r2 < T.rc2lo || (r2 < T.rc2hi && sqrt(r2) < T.rc));
+      if (COUNT) {
+        if (S::lj(s)) w.lj += count_of<UNI>(in_lj);
+        if (S::coul(s)) w.coul += count_of<UNI>(in_c);
+        if (S::dip(s)) w.dip += count_of<UNI>(in_c);
+      }
       if (in_lj || in_c) {
         const double rinv = rsqrt(r2);
         const double r2inv = rinv * rinv;
@@ -304,7 +328,6 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
           a[s].fx += flj * dx;
           a[s].fy += flj * dy;
           a[s].fz += flj * dz;
-          if (COUNT) w.lj++;
         }
         if (in_c) {
           // shifted force r^-3 (1 - r^3/rc^3) = r^-3 - rc^-3 (src/Force.cpp:671-683 and
@@ -318,7 +341,6 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
             a[s].fx += fc * dx;
             a[s].fy += fc * dy;
             a[s].fz += fc * dz;
-            if (COUNT) w.coul++;
           }
           if (S::dip(s)) {
             const double A = at.w * r3s;
@@ -333,12 +355,17 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
             a[s].Exy += bx * dy;
             a[s].Exz += bx * dz;
             a[s].Eyz += by * dz;
-            if (COUNT) w.dip++;
           }
         }
       }
     } else {
       // all-atom kernels (no cutoff, no shifts): src/Force.cpp:117-300, :495-574, :836-897
+      if (COUNT) {
+        const uint32_t c = count_of<UNI>(true);
+        if (S::lj(s)) w.lj += c;
+        if (S::coul(s)) w.coul += c;
+        if (S::dip(s)) w.dip += c;
+      }
       const double rinv = rsqrt(r2);
       const double r2inv = rinv * rinv;
       if (S::lj(s)) {
@@ -348,7 +375,6 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
         a[s].fx += flj * dx;
         a[s].fy += flj * dy;
         a[s].fz += flj * dz;
-        if (COUNT) w.lj++;
       }
       if (S::coul(s)) {
         const double uc = qs[s] * at.w * KC * rinv;
@@ -357,7 +383,6 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
         a[s].fx += fc * dx;
         a[s].fy += fc * dy;
         a[s].fz += fc * dz;
-        if (COUNT) w.coul++;
       }
       if (S::dip(s)) {
         const double r3 = rinv * r2inv;
@@ -372,7 +397,6 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
         a[s].Exy += B * dx * dy;
         a[s].Exz += B * dx * dz;
         a[s].Eyz += B * dy * dz;
-        if (COUNT) w.dip++;
       }
     }
   }
@@ -568,63 +592,78 @@ using LIdx = typename std::conditional<LDS, uint16_t, int>::type;
 template <class S, int L, bool LDS>
 __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, const double c[3],
                            double R, double Rin) {
-  RunGen g;
-  g.mode = 0;
-  g.c[0] = c[0];
-  g.c[1] = c[1];
-  g.c[2] = c[2];
-  g.R2 = R * R;
-  const int ix0 = clampi((int)floor((c[0] - R - T.cx) * T.inv_cell), 0, T.nx - 1);
-  const int ix1 = clampi((int)floor((c[0] + R - T.cx) * T.inv_cell), 0, T.nx - 1);
-  const int iy0 = clampi((int)floor((c[1] - R - T.cy) * T.inv_cell), 0, T.ny - 1);
-  const int iy1 = clampi((int)floor((c[1] + R - T.cy) * T.inv_cell), 0, T.ny - 1);
-  g.ix0 = ix0;
-  g.iy0 = iy0;
-  g.nys = iy1 - iy0 + 1;
+  LIdx<LDS> *lst = static_cast<LIdx<LDS> *>(sh.list);
+  const int cap = T.list_cap;
   const double R2 = R * R, Rin2 = Rin * Rin;
-  int n = 0, nin = 0;
-  // two passes over the same columns: atoms within Rin, then the rest within R
-  for (int pass = 0; pass < 2; pass++) {
-  const double lo2 = pass ? Rin2 : -1.0, hi2 = pass ? R2 : Rin2;
-  if (pass) nin = n;
-  // every lane of the group runs the same number of visits per chunk of L
-  // candidates (walk_runs strides t by L), so the ballot sees the whole group
-  for (int r0 = 0; r0 < (ix1 - ix0 + 1) * g.nys; r0 += L) {
-    int start = 0, len = 0, first = 1;
-    if (r0 + lane < (ix1 - ix0 + 1) * g.nys) g.get(T, r0 + lane, start, len, first);
-    const int incl = gscan_incl<L>(len, lane);
-    const int total = __shfl(incl, L - 1, L);
-    sh.incl[lane] = incl;
-    sh.base[lane] = start - (incl - len);
-    lds_fence();
-    int cur = 0;
-    for (int t0 = 0; t0 < total; t0 += L) {
-      const int t = t0 + lane;
-      bool keep = false;
-      int a = 0;
-      if (t < total) {
-        while (cur < L - 1 && t >= sh.incl[cur]) cur++;
-        a = sh.base[cur] + t;
-        const double4 at = LDS ? sh.atoms[a] : T.atoms[a];
-        const double ex = at.x - c[0], ey = at.y - c[1], ez = at.z - c[2];
-        const double d2 = ex * ex + ey * ey + ez * ez;
-        keep = d2 >= lo2 && d2 < hi2;
-      }
-      int below, cnt;
-      group_ballot<L>(keep, lane, below, cnt);
-      if (keep && n + below < T.list_cap) static_cast<LIdx<LDS> *>(sh.list)[n + below] = (LIdx<LDS>)a;
-      n += cnt;
+  int nin = 0, nout = 0;
+  // one pass: atoms within Rin fill the list from the front, the others
+  // within R from the back (entry t >= nin is at cap - 1 - (t - nin))
+  auto take = [&](bool valid, int a) {
+    bool kin = false, kout = false;
+    if (valid) {
+      const double4 at = LDS ? sh.atoms[a] : T.atoms[a];
+      const double ex = at.x - c[0], ey = at.y - c[1], ez = at.z - c[2];
+      const double d2 = ex * ex + ey * ey + ez * ez;
+      kin = d2 < Rin2;
+      kout = !kin && d2 < R2;
     }
-    lds_fence();
-  }
+    int bin, cin, bout, cout;
+    group_ballot<L>(kin, lane, bin, cin);
+    group_ballot<L>(kout, lane, bout, cout);
+    if (kin && nin + bin < cap) lst[nin + bin] = (LIdx<LDS>)a;
+    if (kout && nout + bout < cap) lst[cap - 1 - (nout + bout)] = (LIdx<LDS>)a;
+    nin += cin;
+    nout += cout;
+  };
+  if (LDS) {
+    // a target staged in LDS (at most a few thousand atoms): every atom, in
+    // index (cell) order; every lane of the group runs the same iterations
+    for (int t0 = 0; t0 < T.natoms; t0 += L) take(t0 + lane < T.natoms, t0 + lane);
+  } else {
+    // the cell columns that the sphere of radius R around c reaches
+    RunGen g;
+    g.mode = 0;
+    g.c[0] = c[0];
+    g.c[1] = c[1];
+    g.c[2] = c[2];
+    g.R2 = R2;
+    const int ix0 = clampi((int)floor((c[0] - R - T.cx) * T.inv_cell), 0, T.nx - 1);
+    const int ix1 = clampi((int)floor((c[0] + R - T.cx) * T.inv_cell), 0, T.nx - 1);
+    const int iy0 = clampi((int)floor((c[1] - R - T.cy) * T.inv_cell), 0, T.ny - 1);
+    const int iy1 = clampi((int)floor((c[1] + R - T.cy) * T.inv_cell), 0, T.ny - 1);
+    g.ix0 = ix0;
+    g.iy0 = iy0;
+    g.nys = iy1 - iy0 + 1;
+    // every lane of the group runs the same number of visits per chunk of L
+    // candidates, so the ballots see the whole group
+    for (int r0 = 0; r0 < (ix1 - ix0 + 1) * g.nys; r0 += L) {
+      int start = 0, len = 0, first = 1;
+      if (r0 + lane < (ix1 - ix0 + 1) * g.nys) g.get(T, r0 + lane, start, len, first);
+      const int incl = gscan_incl<L>(len, lane);
+      const int total = __shfl(incl, L - 1, L);
+      sh.incl[lane] = incl;
+      sh.base[lane] = start - (incl - len);
+      lds_fence();
+      int cur = 0;
+      for (int t0 = 0; t0 < total; t0 += L) {
+        const int t = t0 + lane;
+        int a = 0;
+        if (t < total) {
+          while (cur < L - 1 && t >= sh.incl[cur]) cur++;
+          a = sh.base[cur] + t;
+        }
+        take(t < total, a);
+      }
+      lds_fence();
+    }
   }
   vl.nin = nin;
   vl.c0[0] = c[0];
   vl.c0[1] = c[1];
   vl.c0[2] = c[2];
   vl.R = R;
-  vl.n = n;
-  vl.valid = n <= T.list_cap;
+  vl.n = nin + nout;
+  vl.valid = nin + nout <= cap;
 }
 
 // Forces and energies of all sites at positions p (group-uniform result).
@@ -645,7 +684,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     double4 at;
     double2 lp, lc;
     fetch<LDS, CEN>(T, sh, ai, at, lp, lc);
-    pair_terms<S, COUNT>(T, p, qs, at, lp, lc, first, a, w);
+    pair_terms<S, COUNT, L == 64>(T, p, qs, at, lp, lc, first, a, w);
   };
   RunGen g;
   if (!S::CUT) {
@@ -700,16 +739,19 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       const int n = vl->n, nin = vl->nin;
       if (vl->valid > 0) {
         const LIdx<LDS> *lst = static_cast<const LIdx<LDS> *>(sh.list);
+        // entry t of the list: the front part below nin, the back part after
+        const int back = T.list_cap - 1 + nin;
+        auto entry = [&](int t) { return (int)lst[t < nin ? t : back - t]; };
         // software-pipelined: the next atom's data is read while this one's
         // terms are computed (hides the LDS/L2 latency of the gather)
         double4 at = make_double4(0.0, 0.0, 0.0, 0.0);
         double2 lp = make_double2(0.0, 0.0), lc = make_double2(0.0, 0.0);
-        if (lane < n) fetch<LDS, CEN>(T, sh, (int)lst[lane], at, lp, lc);
+        if (lane < n) fetch<LDS, CEN>(T, sh, entry(lane), at, lp, lc);
         for (int t = lane; t < n; t += L) {
           const double4 cat = at;
           const double2 clp = lp, clc = lc;
-          if (t + L < n) fetch<LDS, CEN>(T, sh, (int)lst[t + L], at, lp, lc);
-          pair_terms<S, COUNT>(T, p, qs, cat, clp, clc, t < nin, a, w);
+          if (t + L < n) fetch<LDS, CEN>(T, sh, entry(t + L), at, lp, lc);
+          pair_terms<S, COUNT, L == 64>(T, p, qs, cat, clp, clc, t < nin, a, w);
         }
         reduce<S, L>(a, sh.red);
         finish<S>(T, a, -1, f, U);
@@ -763,7 +805,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       double4 at;
       double2 lp, lc;
       fetch<LDS, CEN>(T, sh, ai, at, lp, lc);
-      pair_terms<S, COUNT>(T, pp, qs, at, lp, lc, first, b, w);
+      pair_terms<S, COUNT, L == 64>(T, pp, qs, at, lp, lc, first, b, w);
     };
     walk_runs<L>(T, sh, lane, g, nr, visit1);
     reduce<S, L>(b, sh.red);
@@ -973,10 +1015,10 @@ __device__ __forceinline__ void write_record(mccs_traj_record *rec, uint64_t j, 
                                              double bmax, const Work &w, bool count) {
   uint32_t v = 0, l = 0, c = 0, d = 0;
   if (count) {
-    v = gsum_u<L>(w.visits);
-    l = gsum_u<L>(w.lj);
-    c = gsum_u<L>(w.coul);
-    d = gsum_u<L>(w.dip);
+    v = wtotal<L>(w.visits);
+    l = wtotal<L>(w.lj);
+    c = wtotal<L>(w.coul);
+    d = wtotal<L>(w.dip);
   }
   if (lane == 0) {
     mccs_traj_record r;
@@ -1662,10 +1704,10 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
     // the try has ended
     uint32_t wv = 0, wl = 0, wc = 0, wd = 0;
     if (COUNT) {
-      wv = gsum_u<L>(w.visits);
-      wl = gsum_u<L>(w.lj);
-      wc = gsum_u<L>(w.coul);
-      wd = gsum_u<L>(w.dip);
+      wv = wtotal<L>(w.visits);
+      wl = wtotal<L>(w.lj);
+      wc = wtotal<L>(w.coul);
+      wd = wtotal<L>(w.dip);
     }
     if (lane == 0) {
       if (COUNT) {
