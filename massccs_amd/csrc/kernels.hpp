@@ -62,7 +62,8 @@ constexpr int MAX_STEPS = 4000000;                           // per try
 constexpr int TRAJ_BLOCK = MCCS_TRAJ_BLOCK;
 constexpr int TRAJ_WAVES = MCCS_TRAJ_WAVES;
 
-enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4 };
+// ST_HALF: a step whose first half was made at the end of the previous one
+enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4, ST_HALF = 5 };
 
 // Kernel argument: target, grid and run constants.
 struct Tgt {
@@ -831,6 +832,13 @@ __device__ __forceinline__ double anglevec_nb(const double *a, const double *b, 
   na = sqrt(dot3(a, a));
   const double arg = dot3(a, b) / (na * nb);
   if (fabs(arg) > 1.0) return 0.0;
+  if (arg > 0.99995) {
+    // a step's angle is below 0.01 rad: acos x = 2 asin(sqrt((1 - x) / 2)) with
+    // the asin series to y^9 (y <= 0.005, truncation below 1e-24 relative;
+    // 1 - x is exact here)
+    const double y = sqrt((1.0 - arg) * 0.5), y2 = y * y;
+    return 2.0 * (y + y * y2 * (1.0 / 6.0 + y2 * (3.0 / 40.0 + y2 * (5.0 / 112.0 + y2 * (35.0 / 1152.0)))));
+  }
   return acos(arg);
 }
 
@@ -1476,7 +1484,7 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         st.cont = 0;
       }
       status = ST_INIT;
-    } else {  // ST_STEP: first half
+    } else if (status == ST_STEP) {  // first half (the first step of a try)
       const Tgt &T = tg[cur()];
       const double dt = st.dt;
       double rcm[3];
@@ -1619,16 +1627,23 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
       lds_fence();
       continue;
     }
-    // ST_STEP: second half
+    // ST_STEP / ST_HALF: second half
     double vcm[3], Ek = 0.0;
+    // the integrated particles (the atom, or the constrained pair) and their
+    // forces stay in registers for the next step's first half
+    double ri[3], rj[3], vi[3], vj[3], fi[3], fj[3];
+    const double dt = st.dt, hdm = st.hdm;
+    const Rat rc = st.rc;
     if (S::NS == 1) {
-      const double hdm = st.hdm;
-      for (int d = 0; d < 3; d++) vcm[d] = st.vel[0][d] + hdm * st.fs[0][d] * KCONV;
-      if (lane == 0)
-        for (int d = 0; d < 3; d++) st.vel[0][d] = vcm[d];
+      for (int d = 0; d < 3; d++) {
+        fi[d] = st.fs[0][d];
+        ri[d] = st.pos[0][d];
+        vcm[d] = st.vel[0][d] + hdm * fi[d] * KCONV;
+        vi[d] = vcm[d];
+      }
       Ek = kinetic(T.mu, vcm);
     } else {
-      double fs[3][3], ri[3], rj[3], vi[3], vj[3], fi[3], fj[3];
+      double fs[3][3];
       for (int s = 0; s < 3; s++)
         for (int d = 0; d < 3; d++) fs[s][d] = st.fs[s][d];
       for (int d = 0; d < 3; d++) {
@@ -1636,16 +1651,8 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         vi[d] = st.vel[PA][d]; vj[d] = st.vel[PB][d];
       }
       pair_forces<S, GK>(fs, fi, fj);
-      const Rat rc = st.rc;
       rattle2(ri, rj, vi, vj, fi, fj, rc, MA, MB);
       for (int d = 0; d < 3; d++) vcm[d] = 0.5 * (vi[d] + vj[d]);
-      if (lane == 0)
-        for (int d = 0; d < 3; d++) {
-          st.vel[PA][d] = vi[d];
-          st.vel[PB][d] = vj[d];
-          st.vcm[d] = vcm[d];
-          if (GK == 3) st.vel[1][d] = vcm[d];
-        }
       // exit-test kinetic energy: N2 over its two atoms, CO2 over all three
       // sites with the carbon moving at vcm (src/System.cpp:880-886, :1323-1329)
       Ek = kinetic(MA, vi) + kinetic(MB, vj);
@@ -1695,12 +1702,53 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
       }
       if (__shfl(moot, 0, L)) code = TRY_ABANDON;
     }
-    lds_fence();
     if (code == TRY_PENDING) {
-      if (lane == 0) st.steps = steps;
+      // the first half of the next step, from the registers
+      double rn[3];
+      if (S::NS == 1) {
+        for (int d = 0; d < 3; d++) {
+          vi[d] = vi[d] + hdm * fi[d] * KCONV;
+          ri[d] = ri[d] + dt * vi[d];
+          rn[d] = ri[d];
+        }
+      } else {
+        rattle1(ri, rj, vi, vj, fi, fj, rc, dt, T.d2);
+        for (int d = 0; d < 3; d++) {
+          rn[d] = 0.5 * (ri[d] + rj[d]);
+          vcm[d] = 0.5 * (vi[d] + vj[d]);
+        }
+      }
+      double na;
+      const double dth = st.dtheta + anglevec_nb(rn, rcm, st.rnorm, na);
+      lds_fence();
+      if (lane == 0) {
+        constexpr int mid = GK == 2 ? 2 : 1;
+        if (S::NS == 1) {
+          for (int d = 0; d < 3; d++) {
+            st.vel[0][d] = vi[d];
+            st.pos[0][d] = ri[d];
+          }
+        } else {
+          for (int d = 0; d < 3; d++) {
+            st.pos[PA][d] = ri[d]; st.pos[PB][d] = rj[d];
+            st.vel[PA][d] = vi[d]; st.vel[PB][d] = vj[d];
+            st.pos[mid][d] = rn[d];
+            st.vcm[d] = vcm[d];
+          }
+        }
+        st.rnorm = na;
+        st.dtheta = dth;
+        for (int d = 0; d < 3; d++) {
+          st.rold[d] = rn[d];
+          st.rcm[d] = rn[d];
+        }
+        st.steps = steps;
+      }
+      status = ST_HALF;
       lds_fence();
       continue;
     }
+    lds_fence();
     // the try has ended
     uint32_t wv = 0, wl = 0, wc = 0, wd = 0;
     if (COUNT) {
