@@ -282,6 +282,16 @@ __device__ __forceinline__ void fetch(const Tgt &T, const Shm &sh, int a, double
 // Pair terms of one candidate atom against every site.  allow_lj is 0 for
 // second-shell cells of an out-of-grid site (the reference applies LJ on the
 // first shell only).
+// 1/sqrt(x) for x > 0 (a pair distance squared): the hardware estimate with a
+// second-order Newton-Raphson correction, y = y0 + y0 e (1/2 + 3/8 e) with
+// e = 1 - x y0^2 (full FP64 precision), without the special-value handling of
+// the library rsqrt (x is never 0, infinite or NaN here).
+__device__ __forceinline__ double rsq_pos(double x) {
+  const double y0 = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y0, y0, 1.0);
+  return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
 // Work counters: per lane (summed over the group at the end of a try), or,
 // for whole-wavefront groups (UNI), the wavefront's count from a ballot,
 // accumulated uniformly in scalar registers (lane 0 takes part in every
@@ -319,7 +329,7 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
         if (S::dip(s)) w.dip += count_of<UNI>(in_c);
       }
       if (in_lj || in_c) {
-        const double rinv = rsqrt(r2);
+        const double rinv = rsq_pos(r2);
         const double r2inv = rinv * rinv;
         if (S::lj(s) && in_lj) {
           const double r6 = r2inv * r2inv * r2inv;
