@@ -55,6 +55,10 @@ typedef struct {
   int32_t short_range_cutoff, long_range_flag, long_range_cutoff, polarizability;
   int32_t user_ff;          /* 1 if force_field is set */
   int32_t alpha_set;        /* 1 if alpha carries a value */
+  int32_t co2_vcm_fix;      /* CO2 probe: 0 (default) measures chi from the reference's initial
+                               velocity with its x component replaced by y (src/System.cpp:1012);
+                               1 from the true initial velocity (optional key "CO2 vcm fix": "yes";
+                               no counterpart in the reference) */
 } mccs_config;
 
 /* Reference defaults (Input::readInputFile with an input holding only

@@ -40,6 +40,7 @@ class Config(C.Structure):
         ("gas", C.c_int32), ("equipotential", C.c_int32), ("short_range_cutoff", C.c_int32),
         ("long_range_flag", C.c_int32), ("long_range_cutoff", C.c_int32),
         ("polarizability", C.c_int32), ("user_ff", C.c_int32), ("alpha_set", C.c_int32),
+        ("co2_vcm_fix", C.c_int32),
     ]
 
 

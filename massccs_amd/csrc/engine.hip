@@ -324,6 +324,7 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
     T.rlong = longr ? rc : rlj;
   }
   T.skin = 1.0;
+  T.co2fix = P.cfg.co2_vcm_fix != 0;
   E->q.init(E->device);
 
   // launch shape: lanes per trajectory by target size unless given

@@ -254,6 +254,8 @@ void load_json_config(const std::string &path, mccs_config &c, std::string &gas_
     else throw Error(MCCS_ERR_INVALID, "only available the follow gas buffer types: He Ar N2 CO2");
   }
   c.equipotential = yes_no(d, "Equipotential", 0, "need to choice Equipotential: yes or no");
+  // not a reference key: the corrected chi reference of the CO2 probe
+  c.co2_vcm_fix = yes_no(d, "CO2 vcm fix", 0, "need to choice CO2 vcm fix: yes or no");
   c.short_range_cutoff = yes_no(d, "Short-range cutoff", 1, "need to choice Short-range cutoff: yes or no");
   if ((v = member(d, "LJ-cutoff"))) c.lj_cutoff = get_double(*v, "LJ-cutoff");
   c.long_range_flag = yes_no(d, "Long-range forces", 0, "need specify only yes or no for coulomb interactions");

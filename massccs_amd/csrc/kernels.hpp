@@ -88,6 +88,7 @@ struct Tgt {
   double rfar2;             // no term reaches beyond this squared distance
   double rlong;             // longest cutoff of the run's terms
   double skin;              // Verlet-list skin (A)
+  int co2fix;               // CO2: chi from the true initial velocity (not src/System.cpp:1012)
   // far-field surrogate (N2, force type 6): cubic B-spline coefficients of
   // the network's two site energies and of the distance to the nearest atom
   // (x = E_N, y = E_dummy, z = d) on a grid; null when not in use
@@ -1628,7 +1629,7 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         st.hdm = 0.5 * dt / T.mu;
         st.rc = rattle_coeffs(MA, MB, dt);
         for (int d = 0; d < 3; d++) st.vcm_i[d] = st.vcm[d];
-        if (GK == 3) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
+        if (GK == 3 && !T.co2fix) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
         st.dtheta = 0.0;
         for (int d = 0; d < 3; d++) st.rold[d] = st.rcm[d];
         st.rnorm = sqrt(dot3(st.rcm, st.rcm));

@@ -923,7 +923,8 @@ static void run_n2(const oracle_ctx *c, const double pos0[9], const double vel0[
 }
 
 /* System::run_CO2, src/System.cpp:948-1392, including the reference's
- * initial-velocity typo vcm[0] = gasProbe->vcm[1] (:1012). */
+ * initial-velocity typo vcm[0] = gasProbe->vcm[1] (:1012) unless
+ * co2_vcm_fix is set. */
 static void run_co2(const oracle_ctx *c, const double pos0[9], const double vel0[9],
                     const double rcm0[3], const double vcm0[3], oracle_record *o) {
   const oracle_params *p = &c->p;
@@ -943,7 +944,7 @@ static void run_co2(const oracle_ctx *c, const double pos0[9], const double vel0
   (void)p;
   double rcm[3], vcm[3], rcm_i[3], vcm_i[3], rcm_old[3];
   memcpy(rcm, rcm0, sizeof rcm);
-  vcm[0] = vcm0[1];
+  vcm[0] = c->p.co2_vcm_fix ? vcm0[0] : vcm0[1];
   vcm[1] = vcm0[1];
   vcm[2] = vcm0[2];
   memcpy(rcm_i, rcm, sizeof rcm);

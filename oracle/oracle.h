@@ -49,6 +49,8 @@ typedef struct {
   double temperature, dt;
   double a, b, c, bmax;        /* ellipsoid and maximal impact parameter */
   double half_box[3];          /* 0.5 * linked-cell box */
+  int co2_vcm_fix;             /* CO2: 1 measures chi from the true initial velocity
+                                  (0 = the reference, src/System.cpp:1012) */
 } oracle_params;
 
 typedef struct oracle_ctx oracle_ctx;

@@ -46,7 +46,7 @@ class OracleParams(C.Structure):
         ("force_type", C.c_int), ("mu", C.c_double),
         ("temperature", C.c_double), ("dt", C.c_double),
         ("a", C.c_double), ("b", C.c_double), ("c", C.c_double),
-        ("bmax", C.c_double), ("half_box", C.c_double * 3),
+        ("bmax", C.c_double), ("half_box", C.c_double * 3), ("co2_vcm_fix", C.c_int),
     ]
 
 
@@ -144,6 +144,7 @@ class Oracle:
         p.bmax = prep["bmax"]
         for i in range(3):
             p.half_box[i] = prep["halfbox"][i]
+        p.co2_vcm_fix = 1 if cfg.get("CO2 vcm fix", "no") == "yes" else 0
         self.params = p
         self.ctx = lib().oracle_create(C.byref(p))
 

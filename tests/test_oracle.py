@@ -240,3 +240,28 @@ def test_oracle_reproduces_published_logs(key):
         assert float("%g" % mean) == pub["mean"] and float("%g" % err) == pub["err"]
     else:
         assert abs(mean - pub["mean"]) < 1e-4 * pub["mean"]
+
+
+def test_co2_corrected_chi_reference():
+    """The optional corrected CO2 mode ("CO2 vcm fix": "yes") changes only the
+    velocity chi is measured from (the reference replaces its x component by
+    y, src/System.cpp:1012): hits, fates, tries and steps are the same, and a
+    probe whose path never reaches the field keeps its direction (chi = 0 up
+    to the rounding of acos near 1), which the reference's form does not."""
+    from common import C60_XYZ
+    base = dict(targetFileName=C60_XYZ, GasBuffer="CO2", polarizability="yes", seed=2104, **LONG)
+    P = prepared(base)
+    ref = oracle_for(P, base)
+    fix = oracle_for(prepared(dict(base, **{"CO2 vcm fix": "yes"})), dict(base, **{"CO2 vcm fix": "yes"}))
+    u = ref.samples(60)
+    a, b = ref.run(u), fix.run(u)
+    for k in ("hit", "success", "tries", "steps"):
+        assert np.array_equal(a[k], b[k]), k
+    hits = (a["hit"] == 1) & (a["success"] == 1)
+    assert hits.sum() >= 20
+    # the far flights: chi of the corrected form at the rounding floor
+    far = hits & (b["chi"] < 1e-6)
+    assert far.sum() >= 5
+    assert np.all(b["chi"][far] <= 3e-8)
+    # the reference's form reports the angle between (vy, vy, vz) and (vx, vy, vz)
+    assert np.all(a["chi"][far] > 1e-6)
