@@ -836,18 +836,31 @@ __device__ __forceinline__ double anglevec(const double *a, const double *b) {
   if (fabs(arg) > 1.0) return 0.0;
   return acos(arg);
 }
-// The same with |b| given (the previous step's |a|, the same bits) and |a|
-// returned for the next step.
+// 1/x for a positive finite x: the hardware estimate with two Newton-Raphson
+// corrections (within an ulp).
+__device__ __forceinline__ double rcp_pos(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(fma(-x, y, 1.0), y, y);
+  return fma(fma(-x, y, 1.0), y, y);
+}
+
+// The angle between the centre positions of two consecutive steps (the
+// reference's anglevec, src/System.cpp:1566-1579) with |b| given (the previous
+// step's |a|) and |a| returned for the next step.  The square roots and the
+// quotient come from the hardware estimates refined by Newton-Raphson (an ulp
+// or so from the library's); small angles, the usual case, take
+// acos x = 2 asin(sqrt((1 - x) / 2)) with the asin series to y^9 (y <= 0.005:
+// truncation below 1e-24 relative; 1 - x is exact there).
 __device__ __forceinline__ double anglevec_nb(const double *a, const double *b, double nb,
                                               double &na) {
-  na = sqrt(dot3(a, a));
-  const double arg = dot3(a, b) / (na * nb);
+  const double aa = dot3(a, a);
+  na = aa * rsq_pos(aa);
+  const double arg = dot3(a, b) * rcp_pos(na * nb);
   if (fabs(arg) > 1.0) return 0.0;
   if (arg > 0.99995) {
-    // a step's angle is below 0.01 rad: acos x = 2 asin(sqrt((1 - x) / 2)) with
-    // the asin series to y^9 (y <= 0.005, truncation below 1e-24 relative;
-    // 1 - x is exact here)
-    const double y = sqrt((1.0 - arg) * 0.5), y2 = y * y;
+    const double t = (1.0 - arg) * 0.5;
+    if (t == 0.0) return 0.0;
+    const double y = t * rsq_pos(t), y2 = y * y;
     return 2.0 * (y + y * y2 * (1.0 / 6.0 + y2 * (3.0 / 40.0 + y2 * (5.0 / 112.0 + y2 * (35.0 / 1152.0)))));
   }
   return acos(arg);
@@ -1632,7 +1645,8 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         if (GK == 3 && !T.co2fix) st.vcm_i[0] = st.vcm[1];  // src/System.cpp:1012 reads vcm[1] for x
         st.dtheta = 0.0;
         for (int d = 0; d < 3; d++) st.rold[d] = st.rcm[d];
-        st.rnorm = sqrt(dot3(st.rcm, st.rcm));
+        const double aa = dot3(st.rcm, st.rcm);
+        st.rnorm = aa * rsq_pos(aa);  // as anglevec_nb computes |a|
       }
       status = ST_STEP;
       lds_fence();
