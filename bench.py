@@ -299,7 +299,19 @@ def main():
     ap.add_argument("--surrogate", action="store_true",
                     help="ellipsoid50k/shell1m: also build the far-field network and time the "
                          "surrogate path against the plain kernel on the same samples")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 1 GiB float32 tensor three times: the calibration of "
+                         "the FETCH_SIZE / WRITE_SIZE counters for the traffic figure")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 << 28, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the trajectory engine is measured on a single "
               "MI355X; trajectories would shard by index range across GPUs, which is not run here", file=sys.stderr)
