@@ -1328,7 +1328,7 @@ __device__ bool resolve(const Queue &Q, mccs_traj_record *rec, uint64_t j, doubl
   r.coul = c;
   r.dip = d;
   rec[j] = r;
-  __threadfence();
+  // records are read by the host after the launch: no fence before the count
   atomicAdd((unsigned long long *)Q.active, ~0ull);  // -1
   return true;
 }
@@ -1414,10 +1414,7 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
             got = true;
           } else {
             write_record<L>(rec, j, lane, 0, 0, 0, 0, 0.0, tg[tj].bmax, w, COUNT);
-            if (lane == 0) {
-              __threadfence();
-              atomicAdd((unsigned long long *)Q.active, ~0ull);
-            }
+            if (lane == 0) atomicAdd((unsigned long long *)Q.active, ~0ull);
           }
           continue;
         }
@@ -1805,7 +1802,8 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         r.coul = wc;
         r.dip = wd;
         rec[j] = r;
-        __threadfence();
+        // read by the host after the launch: no fence (a release here writes
+        // back the whole L2 of the XCD)
         atomicAdd((unsigned long long *)Q.active, ~0ull);
       } else if (code != TRY_ABANDON) {
         TryOut o;
@@ -1817,7 +1815,7 @@ __global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ
         o.coul = wc;
         o.dip = wd;
         Q.tab[TRIES_MAX * j + k] = o;
-        __threadfence();
+        // the release store publishes the outcome to the other XCDs' groups
         __hip_atomic_store(&Q.tab[TRIES_MAX * j + k].code, code, __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_AGENT);
         // a failed try needs the next one unless the fold is already decided;
