@@ -287,12 +287,13 @@ mccs_status mccs_mlp_train(int32_t device, const mccs_mlp_spec *spec, float *par
 
 typedef struct {
   uint64_t n_train;        /* points sampled by the exact evaluator (default 2e6) */
-  int32_t epochs, batch;   /* default 100, 8192 */
+  int32_t epochs, batch;   /* default 300, 8192 */
   double lr;               /* default 5e-3 (cosine decay to lr / 20) */
   double shell;            /* exact force within this distance of any atom (default 8 A) */
   double blend;            /* exact and surrogate blended over [shell, shell + blend] (2 A) */
   double grid;             /* spacing of the tabulated network (default 0.25 A) */
-  double feature_length;   /* Fourier frequencies ~ N(0, 1 / feature_length^2) per A (18 A) */
+  double feature_length;   /* Fourier frequencies ~ N(0, 1 / feature_length^2) per A; 0 = auto,
+                              0.23 x (largest target extent + 2 x the longest cutoff) */
   uint32_t seed;
   int32_t table_source;    /* 0: the trained network; 1: the exact evaluator at the grid nodes
                               (no network; validates the tabulated path) */

@@ -296,8 +296,8 @@ class Engine:
         _check(lib().mccs_engine_last_work(self.h, out))
         return dict(zip(("visits", "lj", "coul", "dip", "steps"), list(out)))
 
-    def build_surrogate(self, n_train=2_000_000, epochs=100, batch=8192, lr=5e-3, shell=8.0,
-                        blend=2.0, grid=0.25, feature_length=18.0, seed=2104, exact_table=False):
+    def build_surrogate(self, n_train=2_000_000, epochs=300, batch=8192, lr=5e-3, shell=8.0,
+                        blend=2.0, grid=0.25, feature_length=0.0, seed=2104, exact_table=False):
         """Trains and tabulates the far-field network of this (N2, type 6)
         engine's target; returns (report dict, trained parameters, spec).
         exact_table tabulates the exact evaluator instead of the network."""

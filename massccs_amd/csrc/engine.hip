@@ -710,7 +710,13 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
     spec.out_scale[r] = std::max(std::sqrt(ss / (double)ntr), 1e-6);
   }
   std::normal_distribution<double> N01(0.0, 1.0);
-  const double flen = o.feature_length > 0 ? o.feature_length : 18.0;
+  // feature length: the wavelength scale of the random Fourier features,
+  // by default 0.23 of the domain's extent (the target's largest extent plus
+  // the cutoff on both sides): 18-20 A for a 1,200-atom globule, 40 A for
+  // the 50,000-atom ellipsoid, the values that fit those fields best
+  double ext = 0.0;
+  for (int d = 0; d < 3; d++) ext = std::max(ext, T.bhi[d] - T.blo[d]);
+  const double flen = o.feature_length > 0 ? o.feature_length : 0.23 * (ext + 2.0 * T.rlong);
   for (int f = 0; f < MLP_NF; f++)
     for (int d = 0; d < 3; d++) spec.freq[f][d] = (float)(N01(rng) * L / flen);
   std::vector<float> params(MLP_NPARAM, 0.0f);
@@ -764,7 +770,7 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(st));
   TrainOpts to;
-  to.epochs = o.epochs > 0 ? o.epochs : 100;
+  to.epochs = o.epochs > 0 ? o.epochs : 300;
   to.batch = o.batch > 0 ? o.batch : 8192;
   to.lr = o.lr > 0 ? o.lr : 5e-3;
   to.lr_final = to.lr / 20.0;
