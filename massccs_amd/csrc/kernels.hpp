@@ -911,7 +911,7 @@ __device__ __forceinline__ void rot_y(double *r, double ang) {
 
 // System::setup, src/System.cpp:1397-1518.  Returns the hit flag.
 template <class S>
-__device__ bool setup(const Tgt &T, const double *u, double (&pos)[3][3], double (&vel)[3][3],
+__device__ __attribute__((noinline)) bool setup(const Tgt &T, const double *u, double (&pos)[3][3], double (&vel)[3][3],
                       double (&rcm)[3], double (&vcm)[3]) {
   rcm[0] = u[0];
   rcm[1] = 0.0;
@@ -1149,7 +1149,7 @@ __device__ __forceinline__ T_ ld_rlx(const T_ *p) {
 }
 
 // Claims every try of trajectory j not claimed yet and spawns them (lane 0).
-__device__ void spawn_rest(const Queue &Q, uint64_t j) {
+__device__ __attribute__((noinline)) void spawn_rest(const Queue &Q, uint64_t j) {
   if (atomicCAS(Q.spawned + j, 0, 1) != 0) return;
   const int c = atomicExch(Q.tries + j, TRIES_MAX);
   if (c >= TRIES_MAX) return;
@@ -1286,7 +1286,7 @@ __device__ __forceinline__ unsigned long long bcast(unsigned long long v) {
 // Returns true when the trajectory is resolved.  Called by lane 0 after the
 // caller's own outcome is stored; the sequentially consistent fence makes sure
 // that of two tries ending together at least one sees the other's outcome.
-__device__ bool resolve(const Queue &Q, mccs_traj_record *rec, uint64_t j, double bmax) {
+__device__ __attribute__((noinline)) bool resolve(const Queue &Q, mccs_traj_record *rec, uint64_t j, double bmax) {
   __atomic_thread_fence(__ATOMIC_SEQ_CST);
   int tries = 0, max_tries = 8, steps = 0, success = 0, final = -1, capped = 0;
   uint32_t v = 0, l = 0, c = 0, d = 0;
