@@ -6,7 +6,9 @@
 // 64 x 64.  Operand fragments (lane l): A[m = l % 16][k = 4 (l / 16) + e],
 // B[k = 4 (l / 16) + e][n = l % 16], e = 0..3; result D[m = 4 (l / 16) + e]
 // [n = l % 16].  Products whose reduction runs over the batch (weight
-// gradients) are split along k over blocks and summed with FP32 atomics.
+// gradients) are split along k over blocks; each block writes its partial
+// product and the partials are summed in k order, so that training gives the
+// same weights on every run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,8 +72,9 @@ __global__ void features_kernel(const float *__restrict__ xyz, int n, DSpec sp,
 
 // C (M x N, FP32) = op(A) (M x K) * op(B) (K x N), BF16 operands.
 // op(A)[m][k] = TA ? A[k lda + m] : A[m lda + k]; op(B)[k][n] = TB ? B[n ldb + k] : B[k ldb + n].
-// blockIdx.z selects a k-chunk of kchunk (multiple of 16); ATOMIC adds into C.
-template <bool TA, bool TB, bool ATOMIC>
+// blockIdx.z selects a k-chunk of kchunk (multiple of 16); SPLIT writes the
+// chunk's product to slice z of C (slices of M ldc floats).
+template <bool TA, bool TB, bool SPLIT>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const uint16_t *__restrict__ A,
                                                    int lda, const uint16_t *__restrict__ B, int ldb,
                                                    float *__restrict__ C, int ldc, int kchunk) {
@@ -81,6 +84,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
   const int kb = blockIdx.z * kchunk;
   const int ke = min(K, kb + kchunk);
   const int r16 = lane % 16, q = lane / 16;
+  if (SPLIT) C += (size_t)blockIdx.z * M * ldc;
   f4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -124,11 +128,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
       for (int e = 0; e < 4; e++) {
         const int m = m0 + 16 * i + 4 * q + e;
         const int nn = n0 + 16 * j + r16;
-        if (m < M && nn < N) {
-          if (ATOMIC) atomicAdd(C + (size_t)m * ldc + nn, acc[i][j][e]);
-          else C[(size_t)m * ldc + nn] = acc[i][j][e];
-        }
+        if (m < M && nn < N) C[(size_t)m * ldc + nn] = acc[i][j][e];
       }
+}
+
+// out[i] += the sum of part[z][i] over z = 0 .. nz - 1, in that order.
+__global__ void splitsum_kernel(const float *__restrict__ part, int nz, int n, float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < nz; z++) s += part[(size_t)z * n + i];
+  out[i] += s;
 }
 
 // Z += bias; H = bf16(SiLU(Z)).
@@ -386,6 +396,7 @@ struct Mlp {
   uint16_t *dZ1b = nullptr, *dZ2b = nullptr, *dZ3b = nullptr;
   int *idx = nullptr;
   double *loss = nullptr;
+  float *part = nullptr;  // partial products of the split weight-gradient products
   template <class T>
   T *alloc(size_t count) {
     void *p = nullptr;
@@ -404,15 +415,29 @@ namespace {
 
 unsigned blocks_for(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-template <bool TA, bool TB, bool AT>
+constexpr int MAX_SPLIT = 64;  // k-chunks of a split product
+
+// C = op(A) op(B).  Split (part != null): C += op(A) op(B), the k range cut
+// into at most MAX_SPLIT chunks whose products go to part (MAX_SPLIT M N
+// floats) and are summed in chunk order; C then has ldc == N.
+template <bool TA, bool TB>
 void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
-          float *C, int ldc, int splitk = 1) {
+          float *C, int ldc, int splitk = 1, float *part = nullptr) {
   int kchunk = (K + splitk - 1) / splitk;
   kchunk = (kchunk + 15) / 16 * 16;
   splitk = (K + kchunk - 1) / kchunk;
+  if (!part) {
+    dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
+    hipLaunchKernelGGL((gemm_kernel<TA, TB, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
+                       C, ldc, K);
+    return;
+  }
+  if (splitk > MAX_SPLIT || ldc != N) throw Error(MCCS_ERR_INVALID, "split product outside its workspace");
   dim3 grid((N + 63) / 64, (M + 63) / 64, splitk);
-  hipLaunchKernelGGL((gemm_kernel<TA, TB, AT>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C,
-                     ldc, kchunk);
+  hipLaunchKernelGGL((gemm_kernel<TA, TB, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
+                     part, N, kchunk);
+  hipLaunchKernelGGL(splitsum_kernel, dim3(blocks_for((size_t)M * N)), dim3(256), 0, s,
+                     (const float *)part, splitk, M * N, C);
 }
 
 // Forward over n <= maxb points already in m->pts: leaves Z*, H*, Y.
@@ -420,13 +445,13 @@ void forward(Mlp *m, int n) {
   hipStream_t s = m->stream;
   hipLaunchKernelGGL(features_kernel, dim3(blocks_for((size_t)n * MLP_NF)), dim3(256), 0, s,
                      m->pts, n, m->ds, m->X0);
-  gemm<false, false, false>(s, n, MLP_H, MLP_IN, m->X0, MLP_IN, m->wb + MLP_W1, MLP_H, m->Z1, MLP_H);
+  gemm<false, false>(s, n, MLP_H, MLP_IN, m->X0, MLP_IN, m->wb + MLP_W1, MLP_H, m->Z1, MLP_H);
   hipLaunchKernelGGL(bias_silu_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->Z1,
                      m->w + MLP_B1, n, m->H1);
-  gemm<false, false, false>(s, n, MLP_H, MLP_H, m->H1, MLP_H, m->wb + MLP_W2, MLP_H, m->Z2, MLP_H);
+  gemm<false, false>(s, n, MLP_H, MLP_H, m->H1, MLP_H, m->wb + MLP_W2, MLP_H, m->Z2, MLP_H);
   hipLaunchKernelGGL(bias_silu_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->Z2,
                      m->w + MLP_B2, n, m->H2);
-  gemm<false, false, false>(s, n, MLP_H, MLP_H, m->H2, MLP_H, m->wb + MLP_W3, MLP_H, m->Z3, MLP_H);
+  gemm<false, false>(s, n, MLP_H, MLP_H, m->H2, MLP_H, m->wb + MLP_W3, MLP_H, m->Z3, MLP_H);
   hipLaunchKernelGGL(bias_silu_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->Z3,
                      m->w + MLP_B3, n, m->H3);
   hipLaunchKernelGGL(out_layer_kernel, dim3(blocks_for(n)), dim3(256), 0, s, m->H3, m->wb + MLP_W4,
@@ -445,24 +470,24 @@ void backward(Mlp *m, int n, bool wgrad, bool xgrad) {
                      m->wb + MLP_W4, m->Z3, n, m->dZf, m->dZ3b);
   if (wgrad) {
     hipLaunchKernelGGL(colsum_kernel, dim3(MLP_H), dim3(256), 0, s, m->dZf, n, MLP_H, m->g + MLP_B3);
-    gemm<true, false, true>(s, MLP_H, MLP_H, n, m->H2, MLP_H, m->dZ3b, MLP_H, m->g + MLP_W3, MLP_H, sk);
+    gemm<true, false>(s, MLP_H, MLP_H, n, m->H2, MLP_H, m->dZ3b, MLP_H, m->g + MLP_W3, MLP_H, sk, m->part);
   }
-  gemm<false, true, false>(s, n, MLP_H, MLP_H, m->dZ3b, MLP_H, m->wb + MLP_W3, MLP_H, m->dH, MLP_H);
+  gemm<false, true>(s, n, MLP_H, MLP_H, m->dZ3b, MLP_H, m->wb + MLP_W3, MLP_H, m->dH, MLP_H);
   hipLaunchKernelGGL(silu_back_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->dH,
                      m->Z2, n, m->dZf, m->dZ2b);
   if (wgrad) {
     hipLaunchKernelGGL(colsum_kernel, dim3(MLP_H), dim3(256), 0, s, m->dZf, n, MLP_H, m->g + MLP_B2);
-    gemm<true, false, true>(s, MLP_H, MLP_H, n, m->H1, MLP_H, m->dZ2b, MLP_H, m->g + MLP_W2, MLP_H, sk);
+    gemm<true, false>(s, MLP_H, MLP_H, n, m->H1, MLP_H, m->dZ2b, MLP_H, m->g + MLP_W2, MLP_H, sk, m->part);
   }
-  gemm<false, true, false>(s, n, MLP_H, MLP_H, m->dZ2b, MLP_H, m->wb + MLP_W2, MLP_H, m->dH, MLP_H);
+  gemm<false, true>(s, n, MLP_H, MLP_H, m->dZ2b, MLP_H, m->wb + MLP_W2, MLP_H, m->dH, MLP_H);
   hipLaunchKernelGGL(silu_back_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->dH,
                      m->Z1, n, m->dZf, m->dZ1b);
   if (wgrad) {
     hipLaunchKernelGGL(colsum_kernel, dim3(MLP_H), dim3(256), 0, s, m->dZf, n, MLP_H, m->g + MLP_B1);
-    gemm<true, false, true>(s, MLP_IN, MLP_H, n, m->X0, MLP_IN, m->dZ1b, MLP_H, m->g + MLP_W1, MLP_H, sk);
+    gemm<true, false>(s, MLP_IN, MLP_H, n, m->X0, MLP_IN, m->dZ1b, MLP_H, m->g + MLP_W1, MLP_H, sk, m->part);
   }
   if (xgrad)
-    gemm<false, true, false>(s, n, MLP_IN, MLP_H, m->dZ1b, MLP_H, m->wb + MLP_W1, MLP_H, m->dX, MLP_IN);
+    gemm<false, true>(s, n, MLP_IN, MLP_H, m->dZ1b, MLP_H, m->wb + MLP_W1, MLP_H, m->dX, MLP_IN);
 }
 
 DSpec device_spec(const MlpSpec &sp) {
@@ -509,6 +534,7 @@ Mlp *mlp_create(int device, const MlpSpec &spec, const float *params, int max_ba
   m->grd = m->alloc<float>(B * 3);
   m->idx = m->alloc<int>(B);
   m->loss = m->alloc<double>(1);
+  m->part = m->alloc<float>((size_t)MAX_SPLIT * MLP_H * MLP_H);
   MLP_OK(hipMemcpy(m->w, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
   MLP_OK(hipMemset(m->g, 0, sizeof(float) * MLP_NPARAM));
   MLP_OK(hipMemset(m->m, 0, sizeof(float) * MLP_NPARAM));

@@ -87,6 +87,32 @@ def test_training_fits_a_smooth_field():
     assert np.sqrt(np.mean((y - ev) ** 2)) <= 0.1 * np.sqrt(np.mean(ev ** 2))
 
 
+def test_training_is_reproducible():
+    """The batch-wide weight-gradient products are split over blocks and the
+    partial products summed in a fixed order: two trainings from the same
+    start give the same weights bit for bit."""
+    print("case mlp train twice", flush=True)
+    p, freq, centre, s, osc, rng = _net(3)
+    xyz = rng.uniform(-30, 30, (20000, 3))
+    e = np.stack([np.sin(xyz[:, 0] / 7.0), np.cos(xyz[:, 2] / 9.0)], 1)
+    spec = M.mlp_spec(freq.astype(np.float32), centre, s, np.sqrt(np.mean(e ** 2, 0)))
+    a, ha = M.mlp_train(spec, p, xyz, e, epochs=3, batch=8192, lr=3e-3)
+    b, hb = M.mlp_train(spec, p, xyz, e, epochs=3, batch=8192, lr=3e-3)
+    assert not np.array_equal(a, p)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_surrogate_build_is_reproducible(workdir):
+    """Two builds of the far-field network for one engine (training points,
+    exact values, training) give the same parameters bit for bit."""
+    print("case surrogate build twice", flush=True)
+    P = prepared(globule_cfg(workdir))
+    E = M.Engine(P)
+    _, pa, _ = E.build_surrogate(n_train=40000, epochs=2)
+    _, pb, _ = E.build_surrogate(n_train=40000, epochs=2)
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+
+
 def _ccs_pair(E, u):
     E.use_surrogate(False)
     E.run(u[:2000])
