@@ -131,7 +131,9 @@ typedef struct mccs_engine mccs_engine;
 typedef struct {
   int32_t device;          /* HIP device ordinal */
   int32_t lanes;           /* lanes cooperating on one trajectory: 0 = auto, 16, 32 or 64 */
-  int32_t block;           /* threads per block: 0 = auto, else 128 or 256 */
+  int32_t block;           /* threads per block: 0 = auto (768 when the target's atoms are
+                              staged in LDS, else 256), else a multiple of 64 from 128 up
+                              to that */
   int32_t use_lds;         /* -1 auto, 0 global-memory atoms, 1 atoms staged in LDS */
   int32_t count_work;      /* 1: fill the work counters of mccs_traj_record */
   int32_t spec_below;      /* tries of a restarting trajectory run one after another; once

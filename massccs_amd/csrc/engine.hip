@@ -333,20 +333,24 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   int L = o.lanes ? o.lanes : 64;
   if (L != 16 && L != 32 && L != 64) throw Error(MCCS_ERR_INVALID, "lanes must be 0, 16, 32 or 64");
   E->lanes = L;
-  E->block = o.block ? o.block : TRAJ_BLOCK;
-  if (E->block % 64 != 0 || E->block < 128 || E->block > TRAJ_BLOCK)
-    throw Error(MCCS_ERR_INVALID, "block must be 0 or a multiple of 64 in [128, " +
-                                      std::to_string(TRAJ_BLOCK) + "]");
   // Atoms (and their LJ pairs) are staged in LDS when the block's share of a
-  // CU's LDS fits them (the CU holds 4 x TRAJ_WAVES wavefronts); Verlet lists
-  // then hold 16-bit indices.
+  // CU's LDS (the CU holds 4 x traj_waves wavefronts) fits them together with
+  // full-size Verlet lists, which then hold 16-bit indices.  The launch shape
+  // follows: traj_block(true) threads when staged, else traj_block(false).
+  if (o.block && (o.block % 64 != 0 || o.block < 128))
+    throw Error(MCCS_ERR_INVALID, "block must be 0 or a multiple of 64, at least 128");
   const bool central = E->gk == 3;
-  const int blocks_per_cu = std::max(1, 4 * TRAJ_WAVES * 64 / E->block);
+  const int lds_block = o.block ? o.block : traj_block(true);
+  const int blocks_per_cu = std::max(1, 4 * traj_waves(true) * 64 / lds_block);
   const size_t per_block = std::min<size_t>(150 * 1024, 160 * 1024 / blocks_per_cu);
-  const bool fits = T.natoms <= 65535 &&
-                    lds_bytes(T.natoms, T.natoms, E->block, L, true, central) <= per_block;
+  const bool fits = T.natoms <= 65535 && lds_block <= traj_block(true) &&
+                    lds_bytes(T.natoms, T.natoms, lds_block, L, true, central) <= per_block;
   E->lds = o.use_lds < 0 ? fits : (o.use_lds != 0);
   if (E->lds && T.natoms > 65535) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
+  E->block = o.block ? o.block : traj_block(E->lds);
+  if (E->block > traj_block(E->lds))
+    throw Error(MCCS_ERR_INVALID, "block must be at most " + std::to_string(traj_block(E->lds)) +
+                                      (E->lds ? " with atoms in LDS" : " with atoms in global memory"));
   // list capacity: every atom for small targets, else what 120 KB of lists allow
   const int groups = E->block / L;
   const size_t lidx = E->lds ? 2 : 4;

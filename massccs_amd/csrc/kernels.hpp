@@ -53,14 +53,12 @@ constexpr int MAX_STEPS = 4000000;                           // per try
 
 // Launch shape of the group kernel: threads per block (the largest block the
 // launcher may use) and wavefronts per SIMD the register budget is sized for.
-#ifndef MCCS_TRAJ_BLOCK
-#define MCCS_TRAJ_BLOCK 256
-#endif
-#ifndef MCCS_TRAJ_WAVES
-#define MCCS_TRAJ_WAVES 2
-#endif
-constexpr int TRAJ_BLOCK = MCCS_TRAJ_BLOCK;
-constexpr int TRAJ_WAVES = MCCS_TRAJ_WAVES;
+// Atoms staged in LDS: 768 threads, one block per CU, so that its 12
+// wavefronts share one copy of the atoms, and 3 waves per SIMD (168 VGPRs).
+// Atoms read through L2: 256 threads at 2 waves per SIMD (231 VGPRs), which
+// leaves each group's Verlet list the LDS of a CU shared by 8 groups.
+__host__ __device__ constexpr int traj_block(bool lds) { return lds ? 768 : 256; }
+__host__ __device__ constexpr int traj_waves(bool lds) { return lds ? 3 : 2; }
 
 // ST_HALF: a step whose first half was made at the end of the previous one
 enum : int { ST_NEED = 0, ST_INIT = 1, ST_RESTART = 2, ST_STEP = 3, ST_DONE = 4, ST_HALF = 5 };
@@ -1338,7 +1336,7 @@ __device__ __attribute__((noinline)) bool resolve(const Queue &Q, mccs_traj_reco
 // every working group performs exactly one force evaluation: the initial
 // force of a try, or the force of one velocity-Verlet/RATTLE step.
 template <int GK, int FT, int L, bool LDS, bool COUNT, bool MT>
-__global__ __launch_bounds__(TRAJ_BLOCK) __attribute__((amdgpu_waves_per_eu(TRAJ_WAVES))) void traj_kernel(
+__global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu(traj_waves(LDS)))) void traj_kernel(
     const Tgt *__restrict__ tg, const unsigned long long *__restrict__ toff, int ntg,
     const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec, Queue Q) {
   static_assert(!(MT && LDS), "targets staged in LDS need a launch of their own");
