@@ -36,11 +36,11 @@ FP64_PEAK_TFLOPS = 78.6
 # gradient (39); the dipole epilogue per site evaluation (27).
 W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
 # HBM bytes per launch of this workload's trajectory kernel, from two
-# rocprofv3 --pmc runs: FETCH_SIZE 37,235 KiB x 2.0 + WRITE_SIZE 1,255,778 KiB
+# rocprofv3 --pmc runs: FETCH_SIZE 235,671 KiB x 2.0 + WRITE_SIZE 8,058,230 KiB
 # x 1.0, each scaled by what the counter reads for a 1 GiB copy
-# (bench.py --copy-probe); the writes are L2 write-backs of the restart queue's
-# release stores; not measured by this script.
-TRAFFIC_BYTES = (37235 * 2 + 1255778) * 1024
+# (bench.py --copy-probe); most of it is the register-spill scratch of the
+# 3-waves-per-SIMD kernel leaving L2; not measured by this script.
+TRAFFIC_BYTES = (235671 * 2 + 8058230) * 1024
 
 
 def workload_cfg(tmp):
