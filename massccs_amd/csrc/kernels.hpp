@@ -221,7 +221,10 @@ __device__ __forceinline__ void group_ballot(bool flag, int lane, int &below, in
   const int gbase = (int)(threadIdx.x % 64) - lane;
   const unsigned long long gm = (L == 64) ? m : ((m >> gbase) & ((1ull << L) - 1ull));
   total = __popcll(gm);
-  below = __popcll(gm & ((1ull << lane) - 1ull));
+  // a whole wavefront counts with mbcnt (no lane mask held in registers)
+  below = (L == 64) ? (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))
+                    : __popcll(gm & ((1ull << lane) - 1ull));
 }
 
 // Bits of a wave ballot that belong to this lane's group, bit i = group lane i.
