@@ -3,6 +3,7 @@
 #include "kernels.hpp"
 #include "mlp.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -196,6 +197,62 @@ struct QueueWS {
   }
 };
 
+// Verlet-list size a probe is likely to need: the 99th percentile, over
+// sampled points of the target's box that a probe's centre can reach (at least
+// dmin from every atom) and that have atoms within R, of the number of atoms
+// within R, counted through the linked cells.  The seed is fixed, so a target
+// always gets the same answer.
+static double list_need(const Prepared &P, double R, double dmin) {
+  if (P.natoms == 0 || P.ncells == 0) return 0.0;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t a = 0; a < P.natoms; a++) {
+    const double v[3] = {P.x[a], P.y[a], P.z[a]};
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], v[d]);
+      hi[d] = std::max(hi[d], v[d]);
+    }
+  }
+  const double R2 = R * R, d2min = dmin * dmin, ic = 1.0 / P.cell;
+  const int n[3] = {P.nx, P.ny, P.nz};
+  std::mt19937_64 rng(0x6c697374ull);
+  std::uniform_real_distribution<double> uni(0.0, 1.0);
+  std::vector<int> counts;
+  for (int smp = 0; smp < 2048; smp++) {
+    double p[3];
+    int c0[3], c1[3];
+    for (int d = 0; d < 3; d++) {
+      p[d] = lo[d] - R + uni(rng) * (hi[d] - lo[d] + 2.0 * R);
+      c0[d] = std::min(std::max((int)std::floor((p[d] - R - P.corner[d]) * ic), 0), n[d] - 1);
+      c1[d] = std::min(std::max((int)std::floor((p[d] + R - P.corner[d]) * ic), 0), n[d] - 1);
+    }
+    int cnt = 0;
+    double m2 = 1e300;
+    for (int i = c0[0]; i <= c1[0]; i++)
+      for (int j = c0[1]; j <= c1[1]; j++)
+        for (int k = c0[2]; k <= c1[2]; k++) {
+          const int64_t c = (int64_t)k + (int64_t)P.nz * j + (int64_t)P.nz * P.ny * i;
+          for (int32_t a = P.cell_start[c]; a < P.cell_start[c + 1]; a++) {
+            const double ex = P.x[a] - p[0], ey = P.y[a] - p[1], ez = P.z[a] - p[2];
+            const double d2 = ex * ex + ey * ey + ez * ez;
+            cnt += d2 < R2;
+            m2 = std::min(m2, d2);
+          }
+        }
+    if (cnt > 0 && m2 >= d2min) counts.push_back(cnt);
+  }
+  if (counts.empty()) return 0.0;
+  std::sort(counts.begin(), counts.end());
+  return counts[(size_t)(0.99 * (double)(counts.size() - 1))];
+}
+
+// LDS a block may use.  A CU has 160 KB: a target staged in LDS takes a whole
+// CU per block; lists of a target read through L2 take half when they fit in
+// it, so that two blocks (2 waves per SIMD) share the CU, else the whole.
+constexpr size_t LDS_BLOCK_FULL = 150 * 1024, LDS_BLOCK_HALF = 78 * 1024;
+// a list radius beyond the cutoff and skin: the largest site offset from the
+// probe's centre (CO2 oxygens, 1.16 A) with margin
+constexpr double LIST_OFF_MAX = 1.2;
+
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -213,6 +270,7 @@ struct Engine {
   bool sur_built = false;
   int spec_below = -1;
   hipStream_t last_stream = nullptr;
+  double list_need = 0.0;  // estimated largest Verlet list at skin 1 A (L2-fed targets)
   int nsites = 1;
   void *dalloc(size_t bytes) {
     void *p = nullptr;
@@ -351,11 +409,18 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   if (E->block > traj_block(E->lds))
     throw Error(MCCS_ERR_INVALID, "block must be at most " + std::to_string(traj_block(E->lds)) +
                                       (E->lds ? " with atoms in LDS" : " with atoms in global memory"));
-  // list capacity: every atom for small targets, else what 120 KB of lists allow
+  // list capacity: every atom for small targets, else what the block's LDS
+  // allows: half a CU when the estimated lists fit in 90 % of it, else all
   const int groups = E->block / L;
   const size_t lidx = E->lds ? 2 : 4;
   const size_t base = lds_bytes(T.natoms, 0, E->block, L, E->lds, central);
-  const size_t room = base < 150 * 1024 ? 150 * 1024 - base : 0;
+  size_t budget = LDS_BLOCK_FULL;
+  if (!E->lds) {
+    E->list_need = list_need(P, T.rlong + T.skin + LIST_OFF_MAX, 2.5);
+    const size_t half = base < LDS_BLOCK_HALF ? (LDS_BLOCK_HALF - base) / (lidx * groups) : 0;
+    if (E->list_need <= 0.9 * (double)half) budget = LDS_BLOCK_HALF;
+  }
+  const size_t room = base < budget ? budget - base : 0;
   T.list_cap = (int)std::min<size_t>((size_t)T.natoms, room / (lidx * groups));
   E->shmem = lds_bytes(T.natoms, T.list_cap, E->block, L, E->lds, central);
   if (E->shmem > 160 * 1024) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
@@ -970,7 +1035,17 @@ static void configure_class(Batch &B, BatchClass &c) {
   const int groups = c.block / c.lanes;
   const bool central = c.gk == 3;
   const size_t base = lds_bytes(0, 0, c.block, c.lanes, false, central);
-  const size_t room = base < 150 * 1024 ? 150 * 1024 - base : 0;
+  // half a CU per block when every member's estimated lists, scaled from a
+  // 1 A skin to the class's, fit in 90 % of it; else the whole CU
+  const size_t half = base < LDS_BLOCK_HALF ? (LDS_BLOCK_HALF - base) / (sizeof(int) * groups) : 0;
+  bool fit_half = true;
+  for (int t : c.members) {
+    const Engine *e = B.eng[t];
+    const double r1 = e->T.rlong + 1.0 + LIST_OFF_MAX, rs = e->T.rlong + c.skin + LIST_OFF_MAX;
+    fit_half = fit_half && e->list_need * (rs / r1) * (rs / r1) * (rs / r1) <= 0.9 * (double)half;
+  }
+  const size_t budget = fit_half ? LDS_BLOCK_HALF : LDS_BLOCK_FULL;
+  const size_t room = base < budget ? budget - base : 0;
   const int cap = (int)(room / (sizeof(int) * groups));
   std::vector<Tgt> tg;
   int maxcap = 0;
