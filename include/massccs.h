@@ -140,6 +140,11 @@ typedef struct {
                               fewer than this many trajectories are unresolved and no new
                               one is left, idle groups also start its later tries side by
                               side (results are the same either way): -1 = auto, 0 = never */
+  int32_t list_cap;        /* Verlet-list entries per trajectory: 0 = auto (what the block's
+                              LDS allows), else at most this many; a list that outgrows it
+                              is replaced by a walk over the cell columns, which differs only
+                              in summation order.  No counterpart in the reference (it has
+                              no lists). */
 } mccs_engine_opts;
 
 mccs_status mccs_engine_create(const mccs_prepared *p, const mccs_engine_opts *opts,

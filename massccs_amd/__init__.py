@@ -58,7 +58,8 @@ class PreparedInfo(C.Structure):
 
 class EngineOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("block", C.c_int32),
-                ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32)]
+                ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32),
+                ("list_cap", C.c_int32)]
 
 
 class BatchOpts(C.Structure):
@@ -260,8 +261,8 @@ class Engine:
     """Device engine on one GPU (target resident in HBM)."""
 
     def __init__(self, prepared, device=0, lanes=0, block=0, use_lds=-1, count_work=True,
-                 spec_below=-1):
-        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below)
+                 spec_below=-1, list_cap=0):
+        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below, list_cap)
         h = C.c_void_p()
         _check(lib().mccs_engine_create(prepared.h, C.byref(o), C.byref(h)))
         self.h, self.prepared = h, prepared

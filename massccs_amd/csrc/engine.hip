@@ -422,6 +422,8 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   }
   const size_t room = base < budget ? budget - base : 0;
   T.list_cap = (int)std::min<size_t>((size_t)T.natoms, room / (lidx * groups));
+  if (o.list_cap < 0) throw Error(MCCS_ERR_INVALID, "list_cap must be >= 0 (0 = auto)");
+  if (o.list_cap > 0) T.list_cap = std::min(T.list_cap, (int)o.list_cap);
   E->shmem = lds_bytes(T.natoms, T.list_cap, E->block, L, E->lds, central);
   if (E->shmem > 160 * 1024) throw Error(MCCS_ERR_INVALID, "target too large to stage in LDS");
   E->d_tg = reinterpret_cast<Tgt *>(E->dalloc(sizeof(Tgt)));

@@ -188,6 +188,23 @@ def test_trajectories_global_memory_atoms(workdir):
     assert np.array_equal(lds.view(np.uint8), rec.view(np.uint8))
 
 
+@pytest.mark.parametrize("use_lds,cap", [(0, 64), (1, 600)])
+def test_trajectories_list_overflow(workdir, use_lds, cap):
+    """Verlet lists that outgrow their capacity: every list at 64 entries,
+    some at 600 of the globule's 1,231 atoms.  The kernel walks the cell
+    columns instead, switching between list and walk within a trajectory."""
+    cfg = globule_cfg(workdir)
+    print("case globule_N2 lds %d list cap %d" % (use_lds, cap), flush=True)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    u = P.samples(200)
+    ref = orc.run(u)
+    rec = M.Engine(P, use_lds=use_lds, list_cap=cap).run(u)
+    big, ok_big, ok_small, flow = _compare_traj(rec, ref, "globule_N2_cap%d" % cap)
+    assert flow >= 0.98
+    assert ok_big.mean() >= 0.95 and ok_small.all()
+
+
 def test_tally_device_matches_host():
     import torch
     cfg = dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", seed=2104, **LONG)
