@@ -222,10 +222,11 @@ class Prepared:
             _check(lib().mccs_prepare_atoms(C.byref(cfg), n, arr, _dp(xyz),
                                             None if qa is None else _dp(qa), C.byref(h)))
         self.h = h
+        self._destroy = lib().mccs_prepared_destroy  # bound now: usable at interpreter exit
 
     def close(self):
         if getattr(self, "h", None):
-            lib().mccs_prepared_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -266,10 +267,11 @@ class Engine:
         h = C.c_void_p()
         _check(lib().mccs_engine_create(prepared.h, C.byref(o), C.byref(h)))
         self.h, self.prepared = h, prepared
+        self._destroy = lib().mccs_engine_destroy  # bound now: usable at interpreter exit
 
     def close(self):
         if getattr(self, "h", None):
-            lib().mccs_engine_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -335,10 +337,11 @@ class Batch:
         h = C.c_void_p()
         _check(lib().mccs_batch_create(arr, len(self.targets), C.byref(o), C.byref(h)))
         self.h = h
+        self._destroy = lib().mccs_batch_destroy  # bound now: usable at interpreter exit
 
     def close(self):
         if getattr(self, "h", None):
-            lib().mccs_batch_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     __del__ = close
