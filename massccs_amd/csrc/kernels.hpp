@@ -721,12 +721,14 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     near = near || d2 < T.rfar2;
   }
   if (!near) return;
+  // every site's cell index inside the grid: 0 <= floor(u) < n exactly when
+  // 0 <= u < n for an integer n (false for NaN), so no floor is needed
   bool inbox = true;
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
-    const double i = floor((p[s][0] - T.cx) * T.inv_cell);
-    const double j = floor((p[s][1] - T.cy) * T.inv_cell);
-    const double k = floor((p[s][2] - T.cz) * T.inv_cell);
+    const double i = (p[s][0] - T.cx) * T.inv_cell;
+    const double j = (p[s][1] - T.cy) * T.inv_cell;
+    const double k = (p[s][2] - T.cz) * T.inv_cell;
     inbox = inbox && i >= 0.0 && i < T.nx && j >= 0.0 && j < T.ny && k >= 0.0 && k < T.nz;
   }
   if (inbox) {
@@ -737,7 +739,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       const double ex = p[s][0] - p[cs][0], ey = p[s][1] - p[cs][1], ez = p[s][2] - p[cs][2];
       off2 = fmax(off2, ex * ex + ey * ey + ez * ez);
     }
-    const double off = sqrt(off2);
+    const double off = S::NS == 1 ? 0.0 : sqrt(off2);  // one site is the centre (finite here)
     const double c[3] = {p[cs][0], p[cs][1], p[cs][2]};
     if (vl) {
       // rebuild when moved + off + rlong + 1e-9 > R, compared in squares
