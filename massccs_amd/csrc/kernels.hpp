@@ -1565,9 +1565,12 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
     lds_fence();
     const Tgt &T = tg[cur()];
 
-    // ---- phase B: one force evaluation for every working group
+    // ---- phase B: one force evaluation for every working group; positions,
+    // forces and the total energy stay in registers for phase C (the group
+    // holds them uniformly; lane 0 also stores forces and energy to LDS)
+    double p[S::NS][3], f[S::NS][3], Ut = 0.0;
     {
-      double p[S::NS][3], f[S::NS][3], U[S::NS];
+      double U[S::NS];
 #pragma unroll
       for (int s = 0; s < S::NS; s++)
         for (int d = 0; d < 3; d++) p[s][d] = st.pos[s][d];
@@ -1616,7 +1619,6 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
           }
         }
       }
-      double Ut = 0.0;
 #pragma unroll
       for (int s = 0; s < S::NS; s++) Ut += U[s];
       if (lane == 0) {
@@ -1659,23 +1661,20 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
     double ri[3], rj[3], vi[3], vj[3], fi[3], fj[3];
     const double dt = st.dt, hdm = st.hdm;
     const Rat rc = st.rc;
-    if (S::NS == 1) {
+    if constexpr (S::NS == 1) {
       for (int d = 0; d < 3; d++) {
-        fi[d] = st.fs[0][d];
-        ri[d] = st.pos[0][d];
+        fi[d] = f[0][d];
+        ri[d] = p[0][d];
         vcm[d] = st.vel[0][d] + hdm * fi[d] * KCONV;
         vi[d] = vcm[d];
       }
       Ek = kinetic(T.mu, vcm);
     } else {
-      double fs[3][3];
-      for (int s = 0; s < 3; s++)
-        for (int d = 0; d < 3; d++) fs[s][d] = st.fs[s][d];
       for (int d = 0; d < 3; d++) {
-        ri[d] = st.pos[PA][d]; rj[d] = st.pos[PB][d];
+        ri[d] = p[PA][d]; rj[d] = p[PB][d];
         vi[d] = st.vel[PA][d]; vj[d] = st.vel[PB][d];
       }
-      pair_forces<S, GK>(fs, fi, fj);
+      pair_forces<S, GK>(f, fi, fj);
       rattle2(ri, rj, vi, vj, fi, fj, rc, MA, MB);
       for (int d = 0; d < 3; d++) vcm[d] = 0.5 * (vi[d] + vj[d]);
       // exit-test kinetic energy: N2 over its two atoms, CO2 over all three
@@ -1698,7 +1697,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
       code = TRY_ANGLE;
     } else if (fabs(rcm[0]) >= T.hx || fabs(rcm[1]) >= T.hy || fabs(rcm[2]) >= T.hz ||
                outside_ellipsoid(T, rcm)) {
-      const double Ei = st.Ei, Ut = st.Utot;
+      const double Ei = st.Ei;
       const double dE = S::NS == 1 ? fabs(Ut + Ek - Ei) / Ei * 100.0
                                    : fabs((Ut + Ek - Ei) / Ei) * 100.0;
       const double tol = GK == 1 ? 0.5 : (GK == 2 ? 1.0 : 20.0);
