@@ -206,6 +206,31 @@ def config_from_dict(d):
         os.unlink(name)
 
 
+def _symbol_pointers(symbols, n):
+    """NUL-terminated element symbols and a char* per atom for
+    mccs_prepare_atoms.  ASCII symbols (every element symbol) are packed
+    without a Python loop: the UCS-4 code points of a str array become the
+    bytes of one fixed-width buffer; anything else is encoded one by one."""
+    a = np.asarray(symbols)
+    if len(a) != n:
+        raise ValueError("one symbol per atom expected")
+    cp = None
+    if a.dtype.kind == "U" and a.dtype.itemsize > 0:
+        cp = np.ascontiguousarray(a).view(np.uint32).reshape(n, -1)
+        if cp.size and int(cp.max()) >= 128:
+            cp = None
+    elif a.dtype.kind == "S" and a.dtype.itemsize > 0:
+        cp = np.ascontiguousarray(a).view(np.uint8).reshape(n, -1)
+    if cp is not None:
+        w = cp.shape[1] + 1
+        buf = np.zeros((n, w), dtype=np.uint8)
+        buf[:, :w - 1] = cp
+        ptrs = np.uint64(buf.ctypes.data) + np.uint64(w) * np.arange(n, dtype=np.uint64)
+        return (buf, ptrs), ptrs.ctypes.data_as(C.POINTER(C.c_char_p))
+    arr = (C.c_char_p * n)(*[os.fsencode(x) for x in symbols])
+    return arr, arr
+
+
 class Prepared:
     """Prepared target (MoleculeTarget + LinkedCell + ellipsoid) on the host."""
 
@@ -217,10 +242,11 @@ class Prepared:
         else:
             xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
             n = len(xyz)
-            arr = (C.c_char_p * n)(*[os.fsencode(s) for s in symbols])
+            buf, arr = _symbol_pointers(symbols, n)  # buf holds the strings during the call
             qa = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
             _check(lib().mccs_prepare_atoms(C.byref(cfg), n, arr, _dp(xyz),
                                             None if qa is None else _dp(qa), C.byref(h)))
+            del buf
         self.h = h
         self._destroy = lib().mccs_prepared_destroy  # bound now: usable at interpreter exit
 

@@ -172,6 +172,14 @@ def test_prepare_from_memory_equals_file(tmp_path):
     for k, v in P1.atoms().items():
         assert np.array_equal(v, P2.atoms()[k]), k
     assert P1.info.bmax == P2.info.bmax
+    # symbols as a str array (packed without a Python loop) and as bytes
+    # (encoded one by one) give the same target
+    for sym in (np.asarray(s, dtype=str), [x.encode() for x in s]):
+        P3 = M.Prepared(M.config_from_dict(cfg), symbols=sym, xyz=back[:, :3], q=back[:, 3])
+        for k, v in P1.atoms().items():
+            assert np.array_equal(v, P3.atoms()[k]), k
+    with pytest.raises(ValueError):
+        M.Prepared(M.config_from_dict(cfg), symbols=list(s)[:-1], xyz=back[:, :3], q=back[:, 3])
 
 
 def test_synthetic_generators():
