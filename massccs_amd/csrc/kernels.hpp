@@ -330,7 +330,11 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
         if (S::coul(s)) w.coul += count_of<UNI>(in_c);
         if (S::dip(s)) w.dip += count_of<UNI>(in_c);
       }
-      if (in_lj || in_c) {
+      // Coulomb and dipole terms without a branch: most listed atoms are
+      // inside their cutoff, and outside it the weights (qq, A, B) are zero,
+      // so those lanes add zeros (the terms of the others are unchanged)
+      const bool any_c = S::coul(s) || S::dip(s);
+      if (any_c || in_lj) {
         const double rinv = rsq_pos(r2);
         const double r2inv = rinv * rinv;
         if (S::lj(s) && in_lj) {
@@ -342,13 +346,13 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
           a[s].fy += flj * dy;
           a[s].fz += flj * dz;
         }
-        if (in_c) {
+        if (any_c) {
           // shifted force r^-3 (1 - r^3/rc^3) = r^-3 - rc^-3 (src/Force.cpp:671-683 and
           // the smoothed field of :393-426)
           const double r3 = rinv * r2inv;
           const double r3s = r3 - T.irc3;
           if (S::coul(s)) {
-            const double qq = qs[s] * qk;
+            const double qq = in_c ? qs[s] * qk : 0.0;
             a[s].U += qq * (rinv + T.cc1 + T.cc2 * r2);
             const double fc = qq * r3s;
             a[s].fx += fc * dx;
@@ -356,8 +360,8 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
             a[s].fz += fc * dz;
           }
           if (S::dip(s)) {
-            const double A = at.w * r3s;
-            const double B = m3q * r3 * r2inv;
+            const double A = in_c ? at.w * r3s : 0.0;
+            const double B = in_c ? m3q * r3 * r2inv : 0.0;
             a[s].Ex += dx * A;
             a[s].Ey += dy * A;
             a[s].Ez += dz * A;
