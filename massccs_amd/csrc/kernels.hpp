@@ -334,14 +334,17 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
       // inside their cutoff, and outside it the weights (qq, A, B) are zero,
       // so those lanes add zeros (the terms of the others are unchanged)
       const bool any_c = S::coul(s) || S::dip(s);
-      if (any_c || in_lj) {
+      if (any_c || (S::lj(s) && allow_lj)) {
         const double rinv = rsq_pos(r2);
         const double r2inv = rinv * rinv;
-        if (S::lj(s) && in_lj) {
+        if (S::lj(s) && allow_lj) {
+          // list strides past the inner part skip this as a whole; inside it,
+          // atoms beyond the LJ cutoff get zero coefficients (zero terms)
+          const double l1 = in_lj ? lj1 : 0.0, l2 = in_lj ? lj2 : 0.0;
           const double r6 = r2inv * r2inv * r2inv;
-          const double ucut = T.rc6inv * (lj2 * T.rc6inv - lj1);
-          a[s].U += r6 * (lj2 * r6 - lj1) - ucut;
-          const double flj = r6 * (12.0 * lj2 * r6 - 6.0 * lj1) * r2inv;
+          const double ucut = T.rc6inv * (l2 * T.rc6inv - l1);
+          a[s].U += r6 * (l2 * r6 - l1) - ucut;
+          const double flj = r6 * (12.0 * l2 * r6 - 6.0 * l1) * r2inv;
           a[s].fx += flj * dx;
           a[s].fy += flj * dy;
           a[s].fz += flj * dz;
