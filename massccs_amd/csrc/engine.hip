@@ -197,6 +197,23 @@ struct QueueWS {
   }
 };
 
+// The smallest double t with sqrt(t) >= rc, so that the reference's cutoff
+// test sqrt(r2) < rc holds exactly when r2 < t: the square root is correctly
+// rounded on the host and on the device, hence monotonic, and t lies within a
+// few units in the last place of rc * rc.
+static double sqrt_cut(double rc) {
+  if (!(rc > 0.0)) return 0.0;                      // nothing is inside
+  if (std::isinf(rc)) return rc;                    // everything finite is inside
+  double t = rc * rc;
+  for (int i = 0; i < 8 && std::sqrt(t) < rc; i++) t = std::nextafter(t, HUGE_VAL);
+  for (int i = 0; i < 8; i++) {
+    const double d = std::nextafter(t, 0.0);
+    if (std::sqrt(d) >= rc) t = d;
+    else break;
+  }
+  return t;
+}
+
 // Verlet-list size a probe is likely to need: the 99th percentile, over
 // sampled points of the target's box that a probe's centre can reach (at least
 // dmin from every atom) and that have atoms within R, of the number of atoms
@@ -350,8 +367,8 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   T.iea = 1.0 / P.a; T.ieb = 1.0 / P.b; T.iec = 1.0 / P.c;
   const double rlj = P.cfg.lj_cutoff, rc = P.cfg.coul_cutoff;
   T.rlj = rlj; T.rc = rc;
-  T.rlj2lo = rlj * rlj * (1.0 - 1e-15); T.rlj2hi = rlj * rlj * (1.0 + 1e-15);
-  T.rc2lo = rc * rc * (1.0 - 1e-15); T.rc2hi = rc * rc * (1.0 + 1e-15);
+  T.rlj2cut = sqrt_cut(rlj);
+  T.rc2cut = sqrt_cut(rc);
   T.irc3 = 1.0 / (rc * rc * rc);
   T.rc6inv = 1.0 / std::pow(rlj, 6);
   T.cc1 = -1.5 / rc;

@@ -77,7 +77,7 @@ struct Tgt {
   double hx, hy, hz;        // half box
   double ea, eb, ec, bmax;  // ellipsoid semi-axes, maximal impact parameter
   double iea, ieb, iec;     // their reciprocals (a pre-test of the exit condition only)
-  double rlj, rc, rlj2lo, rlj2hi, rc2lo, rc2hi;
+  double rlj, rc, rlj2cut, rc2cut;  // sqrt(r2) < rlj exactly when r2 < rlj2cut; same for rc
   double irc3, cc1, cc2;    // 1/rc^3, -1.5/rc, 0.5/rc^3
   double rc6inv;            // 1/rlj^6 (LJ energy shift)
   double alpha, mu, dt, temp, ginertia, d2;
@@ -319,12 +319,12 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
     if (COUNT) w.visits += count_of<UNI>(true);
     const double lj1 = S::cen(s) ? lc.x : lp.x, lj2 = S::cen(s) ? lc.y : lp.y;
     if (S::CUT) {
-      // r < cutoff as the reference tests it (sqrt(r2) < rc), decided on r2
-      // except inside a relative band of 1e-15 around the cutoff
+      // r < cutoff as the reference tests it (sqrt(r2) < rc): with a correctly
+      // rounded square root that holds exactly for r2 below the smallest
+      // double whose square root reaches the cutoff (set on the host)
       bool in_lj = false, in_c = false;
-      if (S::lj(s))
-        in_lj = allow_lj && (r2 < T.rlj2lo || (r2 < T.rlj2hi && sqrt(r2) < T.rlj));
-      if (S::coul(s) || S::dip(s)) in_c = (r2 < T.rc2lo || (r2 < T.rc2hi && sqrt(r2) < T.rc));
+      if (S::lj(s)) in_lj = (allow_lj != 0) & (r2 < T.rlj2cut);
+      if (S::coul(s) || S::dip(s)) in_c = r2 < T.rc2cut;
       if (COUNT) {
         if (S::lj(s)) w.lj += count_of<UNI>(in_lj);
         if (S::coul(s)) w.coul += count_of<UNI>(in_c);
