@@ -36,12 +36,12 @@ FP64_PEAK_TFLOPS = 78.6
 # gradient (39); the dipole epilogue per site evaluation (27).
 W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
 # HBM bytes per launch of this workload's trajectory kernel, from two
-# rocprofv3 --pmc runs: FETCH_SIZE 270,626 KiB x 2.0 + WRITE_SIZE 7,734,638 KiB
+# rocprofv3 --pmc runs: FETCH_SIZE 240,881 KiB x 2.0 + WRITE_SIZE 6,716,231 KiB
 # x 1.0, each scaled by what the counter reads for a 1 GiB copy
 # (bench.py --copy-probe); 1.36 GB at 2 waves per SIMD, the rest comes with
 # the 3-wave launch shape (spill scratch, more resident groups); not measured
 # by this script.
-TRAFFIC_BYTES = (270626 * 2 + 7734638) * 1024
+TRAFFIC_BYTES = (240881 * 2 + 6716231) * 1024
 
 
 def workload_cfg(tmp):
