@@ -126,6 +126,90 @@ def pair_scales(P, cfg, r, qi, lj, coul, dip, cut=True):
     return fs + 1e-300, us + 1e-300
 
 
+# ---------------------------------------------------------------- trajectory parity
+# A trajectory is chaotic when its fate or its scattering angle depends on
+# the integration itself.  The oracle runs it twice more: at dt/2 (every
+# try's step halved), and at dt from an impact parameter moved by four units
+# in the last place (4 x 2^-52 relative).  It counts as non-chaotic when the
+# tries and the fate are the same in all three runs and, if it scattered,
+# |chi(dt) - chi(dt/2)| <= CHAOS_DCHI (the step's truncation error stays
+# small) and |chi - chi(b + 4 ulp)| <= CHAOS_ULP_DCHI (a perturbation of the
+# size of a reordered FP64 sum is not amplified).  On a non-chaotic
+# trajectory the rounding of the engine's other summation order cannot move
+# chi by 1e-8 rad; on a chaotic one it moves it by up to O(1).  The ulp run
+# catches the trajectories whose dt/2 change happens to be small although a
+# dt/4 run changes their fate (seen on the 3-site CO2 probe, whose energy
+# test allows 20 %).
+CHAOS_DCHI = 0.1
+CHAOS_ULP_DCHI = 1e-10
+CHI_TOL = 1e-8        # rad, non-chaotic trajectories with chi > CHI_SMALL
+CHI_SMALL = 1e-6      # below: compared as 1 - cos chi
+COS_TOL = 1e-15
+
+_REF_CACHE = {}
+
+
+def oracle_refs(P, cfg, u, key=None):
+    """Oracle records of samples u at the run's dt, at dt/2, and at dt with
+    the impact parameter moved by 4 ulp (cached by key within the test
+    session: several tests compare launch variants against the same
+    trajectories)."""
+    if key is not None and key in _REF_CACHE:
+        return _REF_CACHE[key]
+    orc = oracle_for(P, cfg)
+    ref = orc.run(u)
+    half = oracle_for(P, dict(cfg, dt=float(cfg.get("dt", 10.0)) / 2)).run(u)
+    up = np.array(u, dtype=np.float64, copy=True)
+    up[:, 0] *= 1.0 + 4.0 * 2.0 ** -52
+    pert = orc.run(up)
+    if key is not None:
+        _REF_CACHE[key] = (ref, half, pert)
+    return ref, half, pert
+
+
+def non_chaotic(ref, half, pert):
+    same = ((ref["tries"] == half["tries"]) & (ref["success"] == half["success"]) &
+            (ref["tries"] == pert["tries"]) & (ref["success"] == pert["success"]))
+    # near-zero angles (free flights through the field, acos of 1 - ulp) are
+    # compared as 1 - cos chi, as in the criterion itself
+    big = np.maximum(ref["chi"], pert["chi"]) > CHI_SMALL
+    ulp_ok = np.where(big, np.abs(ref["chi"] - pert["chi"]) <= CHAOS_ULP_DCHI,
+                      np.abs(np.cos(ref["chi"]) - np.cos(pert["chi"])) <= COS_TOL)
+    stable_chi = (ref["success"] == 0) | ((np.abs(ref["chi"] - half["chi"]) <= CHAOS_DCHI) & ulp_ok)
+    return (ref["hit"] == 1) & same & stable_chi
+
+
+def assert_traj_parity(rec, refs, name, min_big=20):
+    """The trajectory criteria against the oracle on the same samples: hit
+    flags identical everywhere; on every non-chaotic hit the same tries and
+    fate; on every non-chaotic scattered trajectory with chi > 1e-6 rad,
+    |dchi| <= 1e-8 rad, and below that |d(1 - cos chi)| <= 1e-15.  No
+    fraction may fail.  Returns the largest |dchi| of the checked set.
+    refs = oracle_refs(...)."""
+    ref, half, pert = refs
+    assert np.array_equal(rec["hit"], ref["hit"]), name
+    nc = non_chaotic(ref, half, pert)
+    fate = (rec["success"] == ref["success"]) & (rec["tries"] == ref["tries"])
+    bad_fate = np.where(nc & ~fate)[0]
+    assert bad_fate.size == 0, (name, "fate differs on non-chaotic trajectories", bad_fate[:10],
+                                rec["tries"][bad_fate[:10]], ref["tries"][bad_fate[:10]])
+    sc = nc & (ref["success"] == 1)
+    big = sc & (ref["chi"] > CHI_SMALL)
+    small = sc & (ref["chi"] <= CHI_SMALL)
+    dchi = np.abs(rec["chi"] - ref["chi"])
+    worst = float(dchi[big].max()) if big.any() else 0.0
+    bad = np.where(big & (dchi > CHI_TOL))[0]
+    assert bad.size == 0, (name, "chi differs", bad[:10], rec["chi"][bad[:10]], ref["chi"][bad[:10]])
+    dcos = np.abs(np.cos(rec["chi"]) - np.cos(ref["chi"]))
+    bad = np.where(small & (dcos > COS_TOL))[0]
+    assert bad.size == 0, (name, "small chi differs", bad[:10])
+    print("%s: hits %d, non-chaotic %d, chi>1e-6 checked %d (max |dchi| %.3g), small chi %d, "
+          "chaotic %d" % (name, int(ref["hit"].sum()), int(nc.sum()), int(big.sum()), worst,
+                          int(small.sum()), int(((ref["hit"] == 1) & ~nc).sum())), flush=True)
+    assert big.sum() >= min_big, (name, "too few non-chaotic scattered trajectories", big.sum())
+    return worst
+
+
 def gpu_available():
     try:
         import torch

@@ -12,6 +12,9 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
+# timing builds (the reference's release flags): the CPU baseline of bench.py
+ORACLE_FAST_SO = {"v3": os.path.join(REPO, "oracle", "liboracle_fast_v3.so"),
+                  "v4": os.path.join(REPO, "oracle", "liboracle_fast_v4.so")}
 REF_DIR = os.path.join(REPO, "oracle", "_ref")
 HARNESS = os.path.join(REF_DIR, "ref_harness")
 GOLDEN = os.path.join(REPO, "tests", "golden")
@@ -59,16 +62,20 @@ class OracleRecord(C.Structure):
 RECORD_DTYPE = np.dtype([("hit", "<i4"), ("success", "<i4"), ("tries", "<i4"),
                          ("steps", "<i4"), ("chi", "<f8"), ("domega", "<f8")])
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
-            subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "oracle"],
+def lib(variant="strict"):
+    """The oracle library: "strict" (the checker, bit-identical to the strict
+    reference build) or a timing build "v3" / "v4" (-O3 -ffast-math, the
+    reference's release flags, for x86-64-v3 / AVX-512 hosts)."""
+    if variant not in _libs:
+        path = ORACLE_SO if variant == "strict" else ORACLE_FAST_SO[variant]
+        if not os.path.exists(path):
+            subprocess.run(["make", "-C", os.path.join(REPO, "oracle"),
+                            "oracle" if variant == "strict" else "fast"],
                            check=True, capture_output=True)
-        L = C.CDLL(ORACLE_SO)
+        L = C.CDLL(path)
         L.oracle_create.restype = C.c_void_p
         L.oracle_create.argtypes = [C.POINTER(OracleParams)]
         L.oracle_destroy.argtypes = [C.c_void_p]
@@ -91,8 +98,8 @@ def lib():
                                           C.POINTER(C.c_int64)]
         for n in ("oracle_rattle_first", "oracle_rattle_second"):
             getattr(L, n).restype = None
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _dp(a):
@@ -122,8 +129,9 @@ def input_defaults(cfg):
 class Oracle:
     """Owns an oracle context built from a prepared target."""
 
-    def __init__(self, prep, cfg):
+    def __init__(self, prep, cfg, variant="strict"):
         d = input_defaults(cfg)
+        self.L = lib(variant)
         self.prep, self.cfg, self.d = prep, cfg, d
         self._keep = []
         arr = lambda k: self._hold(np.ascontiguousarray(prep[k], dtype=np.float64))
@@ -146,7 +154,7 @@ class Oracle:
             p.half_box[i] = prep["halfbox"][i]
         p.co2_vcm_fix = 1 if cfg.get("CO2 vcm fix", "no") == "yes" else 0
         self.params = p
-        self.ctx = lib().oracle_create(C.byref(p))
+        self.ctx = self.L.oracle_create(C.byref(p))
 
     def _hold(self, a):
         self._keep.append(a)
@@ -154,7 +162,7 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "ctx", None):
-            lib().oracle_destroy(self.ctx)
+            self.L.oracle_destroy(self.ctx)
             self.ctx = None
 
     def site_force(self, kernel, r, qi):
@@ -162,31 +170,31 @@ class Oracle:
         f = np.zeros(3)
         U = C.c_double()
         kid = KERNEL_ID[kernel] if isinstance(kernel, str) else kernel
-        lib().oracle_site_force(self.ctx, kid, _dp(r), qi, _dp(f), C.byref(U))
+        self.L.oracle_site_force(self.ctx, kid, _dp(r), qi, _dp(f), C.byref(U))
         return f, U.value
 
     def samples(self, n, seed=None):
         u = np.zeros((n, 9))
         multi = 1 if self.d["gas"] in (2, 3) else 0
         s = self.d["seed"] if seed is None else seed
-        lib().oracle_samples(s, multi, self.prep["bmax"], n, _dp(u))
+        self.L.oracle_samples(s, multi, self.prep["bmax"], n, _dp(u))
         return u
 
     def run(self, u, nthreads=0):
         u = np.ascontiguousarray(u, dtype=np.float64)
         out = np.zeros(len(u), dtype=RECORD_DTYPE)
-        lib().oracle_run(self.ctx, _dp(u), len(u), out.ctypes.data_as(C.c_void_p), nthreads)
+        self.L.oracle_run(self.ctx, _dp(u), len(u), out.ctypes.data_as(C.c_void_p), nthreads)
         return out
 
     def setup(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         pos, vel, rcm, vcm = np.zeros(9), np.zeros(9), np.zeros(3), np.zeros(3)
-        hit = lib().oracle_setup(self.ctx, _dp(u), _dp(pos), _dp(vel), _dp(rcm), _dp(vcm))
+        hit = self.L.oracle_setup(self.ctx, _dp(u), _dp(pos), _dp(vel), _dp(rcm), _dp(vcm))
         return hit, pos, vel, rcm, vcm
 
     def stencil_sums(self):
         a, b = C.c_int64(), C.c_int64()
-        lib().oracle_stencil_sums(self.ctx, C.byref(a), C.byref(b))
+        self.L.oracle_stencil_sums(self.ctx, C.byref(a), C.byref(b))
         return a.value, b.value
 
 

@@ -1,6 +1,6 @@
 """massccs --batch: several inputs in one batched GPU run, reported as the
 reference data set's table; each CCS equals a single-target run of the same
-launch shape."""
+launch shape, and the He rows equal the oracle's tally of the same samples."""
 import csv
 import json
 import subprocess
@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from common import LONG, M, C60_XYZ, UBQ_PQR, gpu_available, prepared
+from common import LONG, M, O, C60_XYZ, UBQ_PQR, gpu_available, oracle_for, prepared
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
@@ -42,3 +42,14 @@ def test_cli_batch_table(tmp_path):
         assert abs(got / mean - 1) <= 1e-5
         assert abs(float(row["err_" + gas]) / err - 1) <= 1e-4
     assert int(rows[1]["natoms"]) == prepared(cfgs["ubq_He.json"]).info.natoms
+    # the He rows against the oracle's tally of the same samples (every He
+    # trajectory of these targets agrees with the oracle: fates identical, chi
+    # within 1e-9 rad), to the table's printed precision
+    for row, key in [(rows[0], "c60_He.json"), (rows[1], "ubq_He.json")]:
+        c = cfgs[key]
+        P = prepared(c)
+        ref = oracle_for(P, c).run(P.samples(c["numberProbe"] * c["nIter"]))
+        _, _, mean, err = O.tally(ref, c["nIter"], c["numberProbe"])
+        print(key, "table", row["ccs_He"], "oracle", mean, err, flush=True)
+        assert abs(float(row["ccs_He"]) / mean - 1) <= 1e-5
+        assert abs(float(row["err_He"]) / err - 1) <= 1e-4

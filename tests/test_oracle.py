@@ -77,15 +77,29 @@ def test_oracle_matches_live_reference_on_fresh_points():
     assert rec["hit"].tolist() == hit.tolist() and rec["chi"].tolist() == chi.tolist()
 
 
+# Every one of the 16 oracle kernels: F = -grad U pins each term family and,
+# for the kernels whose reference reads uninitialised accumulators
+# (coulomb_induced_dipole_iso, src/Force.cpp:836-853; the dipole kernels of
+# force type 3/5), the zero-initialised restatement that no fixture can pin.
 FD_CASES = [
-    ("He", "lennardjones_induced_dipole_LC", 0.0),
+    ("He", "lennardjones", 0.0),
     ("He", "lennardjones_LC", 0.0),
     ("He", "lennardjones_induced_dipole", 0.0),
-    ("N2", "lennardjones_coulomb_LC", -0.4825),
-    ("N2", "coulomb_induced_dipole_iso_LC", 0.965),
+    ("He", "lennardjones_induced_dipole_LC", 0.0),
     ("N2", "lennardjones_coulomb", -0.4825),
+    ("N2", "lennardjones_coulomb_LC", -0.4825),
+    ("N2", "coulomb", 0.965),
+    ("N2", "coulomb_LC", 0.965),
+    ("N2", "coulomb_induced_dipole_iso", 0.965),
+    ("N2", "coulomb_induced_dipole_iso_LC", 0.965),
+    ("CO2", "lennardjones_CO2", 0.6512),
+    ("CO2", "lennardjones_LC_CO2", 0.6512),
+    ("CO2", "lennardjones_coulomb_CO2", 0.6512),
+    ("CO2", "lennardjones_coulomb_LC_CO2", 0.6512),
+    ("CO2", "lennardjones_coulomb_induced_dipole_iso_CO2", 0.6512),
     ("CO2", "lennardjones_coulomb_induced_dipole_iso_LC_CO2", 0.6512),
 ]
+assert sorted(k for _, k, _ in FD_CASES) == sorted(O.KERNELS)
 
 
 @pytest.mark.parametrize("gas,kernel,qi", FD_CASES, ids=[c[1] for c in FD_CASES])
@@ -144,6 +158,68 @@ def test_linked_cell_sum_equals_all_pairs_within_cutoff():
         scale = np.sum(np.abs(fl + fc) * rr) + 1e-12
         assert np.allclose(f, F, rtol=0, atol=1e-12 * scale)
         assert abs(u - U) <= 1e-12 * (np.sum(np.abs(qq / rr)[m25]) * 3 + np.sum(np.abs(l1) / 12.0 ** 6) + 1)
+
+
+ALL_ATOM_DIPOLE = [
+    ("He", "lennardjones_induced_dipole", 0.0, "plain"),
+    ("N2", "coulomb_induced_dipole_iso", 0.965, None),
+    ("CO2", "lennardjones_coulomb_induced_dipole_iso_CO2", 0.6512, "central"),
+]
+
+
+@pytest.mark.parametrize("gas,kernel,qi,lj", ALL_ATOM_DIPOLE, ids=[c[1] for c in ALL_ATOM_DIPOLE])
+def test_all_atom_dipole_kernels_equal_closed_form(gas, kernel, qi, lj):
+    """The no-cutoff induced-dipole kernels (src/Force.cpp:495-574, :836-897,
+    :1637-1725), whose reference reads uninitialised accumulators, against an
+    independent numpy evaluation of the zero-initialised sums:
+    U = sum LJ + q_i K sum q_j / r - alpha/2 |E|^2 with E = sum q_j d / r^3,
+    F = sum LJ force + q_i K sum q_j d / r^3 + alpha (grad E)^T E."""
+    cfg = dict(targetFileName=PUBLISHED["ubq_He"]["targetFileName"], GasBuffer=gas,
+               polarizability="yes", seed=1, **LONG)
+    P = prepared(cfg)
+    orc = oracle_for(P, cfg)
+    a = P.atoms()
+    alpha = P.info.alpha
+    K = 332.06348078020324
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        r = rng.uniform(-40, 40, 3)
+        d = np.stack([r[0] - a["x"], r[1] - a["y"], r[2] - a["z"]], 1)
+        r2 = np.sum(d * d, 1)
+        rr = np.sqrt(r2)
+        if rr.min() < 1.5:
+            continue
+        U = 0.0
+        F = np.zeros(3)
+        scale_f, scale_u = np.zeros(3), 0.0
+        if lj:
+            eps, sig = (a["eps_c"], a["sig_c"]) if lj == "central" else (a["eps"], a["sig"])
+            l1 = 4 * eps * sig ** 6
+            l2 = l1 * sig ** 6
+            U += np.sum(l2 / r2 ** 6 - l1 / r2 ** 3)
+            fl = (12 * l2 / r2 ** 7 - 6 * l1 / r2 ** 4)[:, None] * d
+            F += fl.sum(0)
+            scale_f += np.abs(fl).sum(0)
+            scale_u += np.sum(np.abs(l2 / r2 ** 6) + np.abs(l1 / r2 ** 3))
+        if qi:
+            uc = qi * K * a["q"] / rr
+            U += uc.sum()
+            fc = (uc / r2)[:, None] * d
+            F += fc.sum(0)
+            scale_f += np.abs(fc).sum(0)
+            scale_u += np.abs(uc).sum()
+        E = np.sum((a["q"] / rr ** 3)[:, None] * d, 0)
+        G = np.sum(a["q"][:, None, None] * (np.eye(3)[None] / rr[:, None, None] ** 3
+                   - 3 * d[:, :, None] * d[:, None, :] / rr[:, None, None] ** 5), 0)
+        U -= 0.5 * alpha * E @ E
+        F += alpha * G @ E
+        sE = np.sum(np.abs(a["q"]) / r2)
+        sG = np.sum(4 * np.abs(a["q"]) / rr ** 3)
+        scale_f += alpha * sE * sG * 3
+        scale_u += alpha * sE * sE
+        f, u = orc.site_force(kernel, r, qi)
+        assert np.all(np.abs(f - F) <= 1e-12 * scale_f), (kernel, r, f, F)
+        assert abs(u - U) <= 1e-12 * scale_u, (kernel, r, u, U)
 
 
 def test_rattle_invariants():
@@ -265,3 +341,31 @@ def test_co2_corrected_chi_reference():
     assert np.all(b["chi"][far] <= 3e-8)
     # the reference's form reports the angle between (vy, vy, vz) and (vx, vy, vz)
     assert np.all(a["chi"][far] > 1e-6)
+
+
+def test_trajectory_criterion_catches_planted_mistakes():
+    """The trajectory criterion of the GPU tests (common.assert_traj_parity,
+    chaos classified by the oracle at dt/2) accepts the oracle against
+    itself and rejects a planted mistake: one angle moved by 3e-8 rad, a
+    polarizability 0.1 % off, a Lennard-Jones well 0.1 % off."""
+    from common import C60_XYZ, assert_traj_parity, oracle_refs
+    cfg = dict(targetFileName=C60_XYZ, GasBuffer="He", polarizability="yes", seed=2104, **LONG)
+    P = prepared(cfg)
+    u = P.samples(1000)
+    refs = oracle_refs(P, cfg, u)
+    ref = refs[0]
+    assert_traj_parity(ref.copy(), refs, "oracle vs itself")
+    moved = ref.copy()
+    k = np.where((ref["success"] == 1) & (ref["chi"] > 1e-3))[0][0]
+    moved["chi"][k] += 3e-8
+    with pytest.raises(AssertionError):
+        assert_traj_parity(moved, refs, "one chi moved")
+    i = P.info
+    a = P.atoms()
+    base = dict(natoms=i.natoms, force_type=i.force_type, grid=[i.nx, i.ny, i.nz, i.ncells],
+                alpha=i.alpha, mu=i.mu, abc=[i.a, i.b, i.c], bmax=i.bmax,
+                halfbox=list(i.half_box), **a)
+    for change in ({"alpha": i.alpha * 1.001}, {"eps": a["eps"] * 1.001}):
+        wrong = O.Oracle(dict(base, **change), cfg).run(u)
+        with pytest.raises(AssertionError):
+            assert_traj_parity(wrong, refs, "planted %s" % list(change))
