@@ -175,6 +175,25 @@ mccs_status mccs_run(mccs_engine *e, const double *samples, uint64_t n,
 mccs_status mccs_run_device(mccs_engine *e, const double *samples_dev, uint64_t n,
                             mccs_traj_record *records_dev, void *stream);
 
+/* Same as mccs_run_device, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): no counterpart in the reference.  The call allocates nothing;
+ * the trajectory work queue (counters and per-trajectory try table) lives in
+ * a statically sized device array that the library holds on each GPU until
+ * the process ends (about 24 MiB: a 1.5 KiB counter block plus 364 bytes per
+ * trajectory and 64 KiB of spare spawn slots).  n may be at most
+ * MCCS_NOALLOC_MAX_TRAJECTORIES; larger n returns MCCS_ERR_INVALID (run them
+ * in chunks).  Every call on one GPU shares that array and the run clears
+ * what it reads first: calls that overlap on one GPU (other streams, other
+ * engines, other threads) race and must be serialised by the caller.  The
+ * engine (created beforehand) still owns its target data; samples and
+ * records are the caller's device buffers as in mccs_run_device.
+ * mccs_engine_last_work reports the run's executed work as for
+ * mccs_run_device. */
+#define MCCS_NOALLOC_MAX_TRAJECTORIES 65536
+mccs_status mccs_run_device_noalloc(mccs_engine *e, const double *samples_dev, uint64_t n,
+                                    mccs_traj_record *records_dev, void *stream);
+
 /* Work the device executed in the last mccs_run / mccs_run_device on this
  * engine, every try included (a trajectory that restarts has its later tries
  * run concurrently; tries made moot by an earlier success are abandoned):

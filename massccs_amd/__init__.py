@@ -89,6 +89,7 @@ class SurrogateReport(C.Structure):
 
 
 MLP_NPARAM = 41602
+NOALLOC_MAX_TRAJECTORIES = 65536  # MCCS_NOALLOC_MAX_TRAJECTORIES
 
 RECORD_DTYPE = np.dtype([("hit", "<i4"), ("success", "<i4"), ("tries", "<i4"), ("steps", "<i4"),
                          ("chi", "<f8"), ("domega", "<f8"), ("visits", "<u8"), ("lj", "<u8"),
@@ -103,7 +104,7 @@ EXPORTS = [
     "mccs_tally_device", "mccs_version", "mccs_engine_last_work", "mccs_batch_create",
     "mccs_batch_destroy", "mccs_batch_run", "mccs_batch_autotune", "mccs_batch_shapes",
     "mccs_batch_last_work", "mccs_mlp_eval", "mccs_mlp_loss_grad", "mccs_mlp_train",
-    "mccs_engine_build_surrogate", "mccs_engine_use_surrogate",
+    "mccs_engine_build_surrogate", "mccs_engine_use_surrogate", "mccs_run_device_noalloc",
 ]
 
 _lib = None
@@ -139,6 +140,8 @@ def lib():
     L.mccs_engine_destroy.restype = None
     L.mccs_run.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.c_void_p]
     L.mccs_run_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+    L.mccs_run_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                          C.c_void_p]
     L.mccs_eval_sites.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_engine_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
@@ -311,6 +314,12 @@ class Engine:
     def run_device(self, u_ptr, n, rec_ptr, stream=None):
         _check(lib().mccs_run_device(self.h, C.c_void_p(u_ptr), n, C.c_void_p(rec_ptr),
                                      C.c_void_p(stream) if stream else None))
+
+    def run_device_noalloc(self, u_ptr, n, rec_ptr, stream=None):
+        """mccs_run_device_noalloc: the same run with the work queue in the
+        library's static device workspace (at most NOALLOC_MAX_TRAJECTORIES)."""
+        _check(lib().mccs_run_device_noalloc(self.h, C.c_void_p(u_ptr), n, C.c_void_p(rec_ptr),
+                                             C.c_void_p(stream) if stream else None))
 
     def tally_device(self, rec_ptr, n_iter, n_traj, omega_ptr, counts_ptr, mean_err_ptr,
                      stream=None):

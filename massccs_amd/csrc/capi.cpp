@@ -190,6 +190,14 @@ mccs_status mccs_run_device(mccs_engine *e, const double *samples_dev, uint64_t 
   });
 }
 
+mccs_status mccs_run_device_noalloc(mccs_engine *e, const double *samples_dev, uint64_t n,
+                                    mccs_traj_record *records_dev, void *stream) {
+  return guard([&] {
+    need(e && ((samples_dev && records_dev) || n == 0), "null argument");
+    mccs::engine_run_device_noalloc(e->e, samples_dev, n, records_dev, stream);
+  });
+}
+
 mccs_status mccs_eval_sites(mccs_engine *e, int32_t site, const double *xyz, uint64_t n, double *f,
                             double *U) {
   return guard([&] {

@@ -9,6 +9,7 @@
 #include <map>
 #include <random>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 
@@ -124,6 +125,59 @@ constexpr int QLINE = 32;  // 64-bit words per counter line
 constexpr int QCTL_WORDS = 6 * QLINE;
 constexpr uint64_t SLOT_SLACK = 8192;  // spawn slots beyond 8 per trajectory
 
+// Per trajectory: TRIES_MAX try outcomes, 8 spawn slots, claimed tries,
+// spawned and resolved flags.
+constexpr size_t QUEUE_PER_TRAJ =
+    TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int);
+// Workspace bytes of a queue over n trajectories.
+constexpr size_t queue_ws_bytes(uint64_t n) {
+  return QUEUE_PER_TRAJ * n + sizeof(unsigned long long) * SLOT_SLACK;
+}
+
+// Queue of n trajectories over a counter block ctl (QCTL_WORDS words) and a
+// workspace ws of queue_ws_bytes(n) bytes, both device memory.
+static Queue queue_over(unsigned long long *ctl, char *ws, uint64_t n, long long spec_below,
+                        unsigned long long wait_ticks) {
+  Queue Q;
+  Q.ticket = ctl + 0 * QLINE;
+  Q.shead = ctl + 1 * QLINE;
+  Q.stail = ctl + 2 * QLINE;
+  Q.active = reinterpret_cast<long long *>(ctl + 3 * QLINE);
+  Q.work = ctl + 4 * QLINE;
+  Q.stalled = reinterpret_cast<int *>(ctl + 5 * QLINE);
+  Q.tab = reinterpret_cast<TryOut *>(ws);
+  Q.slots = reinterpret_cast<unsigned long long *>(ws + TRIES_MAX * sizeof(TryOut) * n);
+  Q.nslots = 8 * n + SLOT_SLACK;
+  Q.tries = reinterpret_cast<int *>(Q.slots + Q.nslots);
+  Q.spawned = Q.tries + n;
+  Q.resolved = Q.spawned + n;
+  Q.wait_ticks = wait_ticks;
+  Q.spec_below = spec_below;
+  return Q;
+}
+
+// Throws when a group of the last run over this counter block gave up
+// waiting for work (its records are then incomplete).  The run must have
+// completed.
+static void queue_check_stall(const unsigned long long *ctl) {
+  unsigned long long c = 0;
+  HIP_OK(hipMemcpy(&c, ctl + 5 * QLINE, sizeof c, hipMemcpyDeviceToHost));
+  if ((int)(c & 0xffffffffu) != 0)
+    throw Error(MCCS_ERR_DEVICE, "trajectory work queue stalled; records are incomplete");
+}
+
+static void queue_work(const unsigned long long *ctl, uint64_t out[5]) {
+  HIP_OK(hipMemcpy(out, ctl + 4 * QLINE, sizeof(uint64_t) * 5, hipMemcpyDeviceToHost));
+}
+
+// A group waits for work at most 120 s (any single try takes far less).
+static unsigned long long queue_wait_ticks(int dev) {
+  int khz = 0;
+  HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) khz = 100000;
+  return 120ull * 1000ull * (unsigned long long)khz;
+}
+
 // Device work queue of one launch: counters, each polled one on its own
 // 256-byte line (ticket, shead, stail, active, work[5], stalled), and a
 // per-trajectory workspace grown on demand.
@@ -142,54 +196,24 @@ struct QueueWS {
     device = dev;
     HIP_OK(hipMalloc(&qctl, sizeof(unsigned long long) * QCTL_WORDS));
     HIP_OK(hipMemset(qctl, 0, sizeof(unsigned long long) * QCTL_WORDS));
-    // a group waits for work at most 120 s (any single try takes far less)
-    int khz = 0;
-    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    if (khz <= 0) khz = 100000;
-    wait_ticks = 120ull * 1000ull * (unsigned long long)khz;
+    wait_ticks = queue_wait_ticks(dev);
   }
   // Queue over n trajectories; spawning below spec_below unresolved ones.
   Queue make(uint64_t n, long long spec_below) {
-    // per trajectory: TRIES_MAX try outcomes, 8 spawn slots, claimed tries,
-    // spawned and resolved flags
-    constexpr size_t per_traj =
-        TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int);
     if (n > qcap) {
       if (qws) HIP_OK(hipFree(qws));
       qws = nullptr;
       qcap = 0;
       const uint64_t cap = std::max<uint64_t>(n, 1024);
-      HIP_OK(hipMalloc(&qws, per_traj * cap + sizeof(unsigned long long) * SLOT_SLACK));
+      HIP_OK(hipMalloc(&qws, queue_ws_bytes(cap)));
       qcap = cap;
     }
-    Queue Q;
-    Q.ticket = qctl + 0 * QLINE;
-    Q.shead = qctl + 1 * QLINE;
-    Q.stail = qctl + 2 * QLINE;
-    Q.active = reinterpret_cast<long long *>(qctl + 3 * QLINE);
-    Q.work = qctl + 4 * QLINE;
-    Q.stalled = reinterpret_cast<int *>(qctl + 5 * QLINE);
-    Q.tab = reinterpret_cast<TryOut *>(qws);
-    Q.slots = reinterpret_cast<unsigned long long *>(qws + TRIES_MAX * sizeof(TryOut) * n);
-    Q.nslots = 8 * n + SLOT_SLACK;
-    Q.tries = reinterpret_cast<int *>(Q.slots + Q.nslots);
-    Q.spawned = Q.tries + n;
-    Q.resolved = Q.spawned + n;
-    Q.wait_ticks = wait_ticks;
-    Q.spec_below = spec_below;
-    return Q;
+    return queue_over(qctl, qws, n, spec_below, wait_ticks);
   }
   // Throws when a group of the last run gave up waiting for work (its
   // records are then incomplete).  The run must have completed.
-  void check_stall() const {
-    unsigned long long c = 0;
-    HIP_OK(hipMemcpy(&c, qctl + 5 * QLINE, sizeof c, hipMemcpyDeviceToHost));
-    if ((int)(c & 0xffffffffu) != 0)
-      throw Error(MCCS_ERR_DEVICE, "trajectory work queue stalled; records are incomplete");
-  }
-  void work(uint64_t out[5]) const {
-    HIP_OK(hipMemcpy(out, qctl + 4 * QLINE, sizeof(uint64_t) * 5, hipMemcpyDeviceToHost));
-  }
+  void check_stall() const { queue_check_stall(qctl); }
+  void work(uint64_t out[5]) const { queue_work(qctl, out); }
   ~QueueWS() {
     if (qctl || qws) hipSetDevice(device);
     if (qctl) hipFree(qctl);
@@ -287,6 +311,7 @@ struct Engine {
   bool sur_built = false;
   int spec_below = -1;
   hipStream_t last_stream = nullptr;
+  const unsigned long long *last_ctl = nullptr;  // queue counters of the last run
   double list_need = 0.0;  // estimated largest Verlet list at skin 1 A (L2-fed targets)
   int nsites = 1;
   void *dalloc(size_t bytes) {
@@ -310,6 +335,8 @@ struct Engine {
     if (stream) hipStreamDestroy(stream);
   }
 };
+
+static void noalloc_setup(int device);
 
 Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   int ndev = 0;
@@ -401,6 +428,7 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   T.skin = 1.0;
   T.co2fix = P.cfg.co2_vcm_fix != 0;
   E->q.init(E->device);
+  noalloc_setup(E->device);
 
   // launch shape: lanes per trajectory by target size unless given
   // launch shape: 64 lanes per trajectory unless given (measured fastest from
@@ -462,42 +490,126 @@ static void set_shmem(const void *fn, size_t bytes) {
     HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-void engine_run_device(Engine *E, const double *u_dev, uint64_t n, mccs_traj_record *rec_dev,
-                       void *stream) {
-  if (n == 0) return;
-  HIP_OK(hipSetDevice(E->device));
-  hipStream_t s = stream ? (hipStream_t)stream : E->stream;
-  TrajFn fn = select_traj(E->gk, E->ft, E->lanes, E->lds, E->count);
-  if (!fn) {  // force types without tuned variants run 64 lanes with work counters
+// Launch shape of a run of n trajectories: the kernel, its blocks and the
+// spawning threshold (fewer trajectories than resident groups unresolved,
+// i.e. when groups begin to fall idle, unless given).
+struct TrajLaunch {
+  TrajFn fn = nullptr;
+  uint64_t blocks = 0;
+  long long spec_below = 0;
+};
+
+static TrajLaunch traj_launch(Engine *E, uint64_t n) {
+  TrajLaunch tl;
+  tl.fn = select_traj(E->gk, E->ft, E->lanes, E->lds, E->count);
+  if (!tl.fn) {  // force types without tuned variants run 64 lanes with work counters
     E->lanes = 64;
     E->count = true;
-    fn = select_traj(E->gk, E->ft, 64, E->lds, true);
+    tl.fn = select_traj(E->gk, E->ft, 64, E->lds, true);
   }
-  if (!fn) throw Error(MCCS_ERR_UNSUPPORTED, "no trajectory kernel for this gas / force type");
-  set_shmem((const void *)fn, E->shmem);
+  if (!tl.fn) throw Error(MCCS_ERR_UNSUPPORTED, "no trajectory kernel for this gas / force type");
+  set_shmem((const void *)tl.fn, E->shmem);
   int per_cu = 0;
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, E->block, E->shmem));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)tl.fn, E->block, E->shmem));
   if (per_cu < 1) throw Error(MCCS_ERR_DEVICE, "trajectory kernel does not fit on a CU");
   const uint64_t groups_per_block = E->block / E->lanes;
-  uint64_t blocks = (uint64_t)per_cu * E->cus;
-  blocks = std::min<uint64_t>(blocks, (n + groups_per_block - 1) / groups_per_block);
-  // automatic: spawning starts once fewer trajectories than resident groups
-  // are unresolved, i.e. when groups begin to fall idle
-  Queue Q = E->q.make(n, E->spec_below >= 0 ? E->spec_below : (long long)(blocks * groups_per_block));
+  tl.blocks = std::min<uint64_t>((uint64_t)per_cu * E->cus,
+                                 (n + groups_per_block - 1) / groups_per_block);
+  tl.spec_below = E->spec_below >= 0 ? E->spec_below : (long long)(tl.blocks * groups_per_block);
+  return tl;
+}
+
+static void launch_traj(Engine *E, const TrajLaunch &tl, Queue Q, const double *u_dev, uint64_t n,
+                        mccs_traj_record *rec_dev, hipStream_t s) {
   launch_queue_init(Q, n, s);
   const Tgt *tg = E->d_tg;
   const unsigned long long *toff = nullptr;
   int ntg = 1;
   void *args[] = {&tg, &toff, &ntg, &u_dev, &n, &rec_dev, &Q};
-  HIP_OK(hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(E->block), args, E->shmem, s));
+  HIP_OK(hipLaunchKernel((const void *)tl.fn, dim3((unsigned)tl.blocks), dim3(E->block), args,
+                         E->shmem, s));
   E->last_stream = s;
+}
+
+void engine_run_device(Engine *E, const double *u_dev, uint64_t n, mccs_traj_record *rec_dev,
+                       void *stream) {
+  if (n == 0) return;
+  HIP_OK(hipSetDevice(E->device));
+  hipStream_t s = stream ? (hipStream_t)stream : E->stream;
+  const TrajLaunch tl = traj_launch(E, n);
+  Queue Q = E->q.make(n, tl.spec_below);
+  launch_traj(E, tl, Q, u_dev, n, rec_dev, s);
+  E->last_ctl = E->q.qctl;
+}
+
+// ---------------------------------------------------------------- no-allocation runs
+// The work queue of mccs_run_device_noalloc lives in this statically sized
+// device array (each GPU holds its own copy until the process ends): the
+// counter block, then the per-trajectory workspace of at most
+// MCCS_NOALLOC_MAX_TRAJECTORIES trajectories.  In host code the name is a
+// stand-in, not a device address: hipGetSymbolAddress gives each device's
+// address once (noalloc_setup, at engine creation with the device current),
+// and only that address is handed to kernels.
+constexpr size_t NOALLOC_CTL_BYTES = sizeof(unsigned long long) * QCTL_WORDS;
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    NOALLOC_CTL_BYTES + queue_ws_bytes(MCCS_NOALLOC_MAX_TRAJECTORIES);
+static_assert(NOALLOC_CTL_BYTES % 256 == 0, "workspace must stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+struct ScratchDev {
+  unsigned char *ptr = nullptr;  // the device's address of scratch
+  unsigned long long wait_ticks = 0;
+};
+std::mutex g_scratch_mu;
+std::map<int, ScratchDev> g_scratch;
+}  // namespace
+
+// Resolves the address of scratch on the current device (once per device).
+static void noalloc_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch.count(device)) return;
+  void *ptr = nullptr;
+  HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw Error(MCCS_ERR_DEVICE, "no device address for the run workspace");
+  ScratchDev d;
+  d.ptr = static_cast<unsigned char *>(ptr);
+  d.wait_ticks = queue_wait_ticks(device);
+  g_scratch[device] = d;
+}
+
+void engine_run_device_noalloc(Engine *E, const double *u_dev, uint64_t n,
+                               mccs_traj_record *rec_dev, void *stream) {
+  if (n > MCCS_NOALLOC_MAX_TRAJECTORIES)
+    throw Error(MCCS_ERR_INVALID, "mccs_run_device_noalloc runs at most " +
+                                      std::to_string(MCCS_NOALLOC_MAX_TRAJECTORIES) +
+                                      " trajectories per call (got " + std::to_string(n) + ")");
+  if (n == 0) return;
+  HIP_OK(hipSetDevice(E->device));
+  ScratchDev d;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(E->device);
+    if (it == g_scratch.end()) throw Error(MCCS_ERR_DEVICE, "run workspace not set up on this device");
+    d = it->second;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : E->stream;
+  const TrajLaunch tl = traj_launch(E, n);
+  unsigned long long *ctl = reinterpret_cast<unsigned long long *>(d.ptr);
+  // queue_init (a kernel) clears the counters and every per-trajectory entry
+  // the run reads before the trajectory kernel starts
+  Queue Q = queue_over(ctl, reinterpret_cast<char *>(d.ptr + NOALLOC_CTL_BYTES), n, tl.spec_below,
+                       d.wait_ticks);
+  launch_traj(E, tl, Q, u_dev, n, rec_dev, s);
+  E->last_ctl = ctl;
 }
 
 void engine_last_work(Engine *E, uint64_t out[5]) {
   HIP_OK(hipSetDevice(E->device));
   HIP_OK(hipStreamSynchronize(E->last_stream ? E->last_stream : E->stream));
-  E->q.check_stall();
-  E->q.work(out);
+  const unsigned long long *ctl = E->last_ctl ? E->last_ctl : E->q.qctl;
+  queue_check_stall(ctl);
+  queue_work(ctl, out);
 }
 
 void engine_run_host(Engine *E, const double *u, uint64_t n, mccs_traj_record *rec) {

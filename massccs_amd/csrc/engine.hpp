@@ -16,6 +16,10 @@ void engine_destroy(Engine *e);
 // Trajectories over device-resident samples/records, enqueued on stream.
 void engine_run_device(Engine *e, const double *u_dev, uint64_t n, mccs_traj_record *rec_dev,
                        void *stream);
+// Same with the work queue in the library's static device workspace (no
+// device allocation; at most MCCS_NOALLOC_MAX_TRAJECTORIES trajectories).
+void engine_run_device_noalloc(Engine *e, const double *u_dev, uint64_t n,
+                               mccs_traj_record *rec_dev, void *stream);
 // Same over host buffers (copies in and out, synchronous).
 void engine_run_host(Engine *e, const double *u, uint64_t n, mccs_traj_record *rec);
 // Per-point site forces over host buffers.
