@@ -1,24 +1,31 @@
 """Benchmark of the trajectory-method hot path on one MI355X.
 
-Workload: the 1,231-atom synthetic globule (ellipsoid 18/15/13 A, min spacing
-1 A, C:N:O:H:S = 0.31:0.08:0.10:0.50:0.01, net +4 e) with the N2 probe,
-LJ + Coulomb + ion-induced dipole with cutoffs 12/25 A (force type 6),
-10 x 10^4 trajectories, seed 2104, dt 10 fs, 298 K.
+Default workload (the configuration the headline metric is quoted on): the
+50,000-atom synthetic ellipsoid (semi-axes 60/45/35 A, net +20 e, C:N:O:H:S
+= 0.31:0.08:0.10:0.50:0.01, min spacing 1 A) with the N2 probe, LJ + Coulomb
++ ion-induced dipole with cutoffs 12/25 A (force type 6), 10^5 trajectories,
+seed 2104, dt 10 fs, 298 K; samples resident in HBM.
 
-One step = the whole Monte-Carlo CCS calculation on the GPU: every
-trajectory of the 10^5 samples (setup, velocity-Verlet/RATTLE, site forces)
-plus the tally, with the samples already resident in HBM.
+One step = the whole Monte-Carlo CCS calculation of the 10^5 trajectories on
+the GPU (setup, velocity-Verlet/RATTLE, site forces, tally).  Two legs:
 
-Printed: one JSON line with trajectories/s, the roofline of the trajectory
-kernel (FP64 work counted by the kernel itself, over its HIP-event duration)
-and the reference CPU program timed on this host's cores.
+* plain: the exact FP64 engine (mccs_run_device + mccs_tally_device);
+* surrogate+batched (the line's `value`): the same target as a batch of one
+  (mccs_batch_run_device: the persistent launch replayed from its hipGraph)
+  with the range-split far field -- pairs nearer than 6 A exact, the
+  remainder of the electrostatic terms beyond them from a cubic B-spline
+  table of the far field -- and mccs_tally_device.
+
+The line carries both rates, their ratio, the CCS error of the surrogate leg
+against the plain leg on the same samples with its paired standard error,
+the roofline of the surrogate leg's trajectory kernel (counted FP64 work
+over its HIP-event time) and the CPU baseline: this project's OpenMP C
+restatement of the reference path at the reference's release flags, timed
+over the reference's CCS-time window on this host's cores (oracle/).
 """
 import argparse
 import json
 import os
-import re
-import shutil
-import subprocess
 import sys
 import tempfile
 import time
@@ -33,32 +40,46 @@ FP64_PEAK_TFLOPS = 78.6
 # FP64 operations per unit of counted work (FMA = 2, sqrt = 1, div = 1):
 # every (site, atom) pair examined: difference vector, r^2, cutoff test (9);
 # LJ term (18); shifted Coulomb (19); smoothed induced dipole field and
-# gradient (39); the dipole epilogue per site evaluation (27).
+# gradient (39); the dipole epilogue per site evaluation (27).  In the
+# range-split kernel the Coulomb / dipole counts are the pairs within the
+# switch radius and the far-field interpolation is not counted.
 W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
-# HBM bytes per launch of this workload's trajectory kernel, from two
-# rocprofv3 --pmc runs: FETCH_SIZE 240,881 KiB x 2.0 + WRITE_SIZE 6,716,231 KiB
-# x 1.0, each scaled by what the counter reads for a 1 GiB copy
-# (bench.py --copy-probe); 1.36 GB at 2 waves per SIMD, the rest comes with
-# the 3-wave launch shape (spill scratch, more resident groups); not measured
-# by this script.
-TRAFFIC_BYTES = (240881 * 2 + 6716231) * 1024
+# Range-split far field of the headline leg: exact below 6 A, switched to
+# zero at 12 A, far field on a 0.5 A grid from the exact evaluator.
+SUR_SHELL, SUR_BLEND, SUR_GRID = 6.0, 6.0, 0.5
 
 
-def workload_cfg(tmp):
+def n2_cfg(path="memory.xyz"):
+    return {"targetFileName": path, "GasBuffer": "N2", "Short-range cutoff": "yes",
+            "LJ-cutoff": 12.0, "Long-range forces": "yes", "Long-range cutoff": "yes",
+            "Coul-cutoff": 25.0, "polarizability": "yes", "alpha": 1.7403, "seed": 2104,
+            "dt": 10.0, "Temp": 298.0}
+
+
+def ellipsoid50k():
+    from massccs_amd import synth
+    s, xyz, q = synth.globule(n=50000, axes=(60.0, 45.0, 35.0), net_charge=20.0, seed=2104)
+    return n2_cfg(), (s, xyz, q), "50,000-atom ellipsoid 60/45/35 A, net +20 e"
+
+
+def shell1m():
+    from massccs_amd import synth
+    s, xyz, q = synth.shell()
+    return n2_cfg(), (s, xyz, q), "1,000,000-atom hollow shell r 120-140 A, net +60 e"
+
+
+def globule_cfg(tmp):
     from massccs_amd import synth
     path = os.path.join(tmp, "globule_1231.xyz")
     s, xyz, q = synth.globule(n=1231, axes=(18.0, 15.0, 13.0), net_charge=4.0, seed=2104)
     synth.write_xyz(path, s, xyz, q, comment="synthetic globule 1231 atoms, net +4e")
-    return {"targetFileName": path, "GasBuffer": "N2", "Short-range cutoff": "yes",
-            "LJ-cutoff": 12.0, "Long-range forces": "yes", "Long-range cutoff": "yes",
-            "Coul-cutoff": 25.0, "polarizability": "yes", "alpha": 1.7403, "seed": 2104,
-            "dt": 10.0, "Temp": 298.0, "numberProbe": 10000, "nIter": 10}
+    return dict(n2_cfg(path), numberProbe=10000, nIter=10)
 
 
 def small_cfg(tmp, which):
-    """Other single-target workloads: the C60 Lennard-Jones case the reference
-    runs on its CPU (He, force type 2, 3 x 10^4 trajectories) and the
-    published ubiquitin / He configuration (force type 4, 10 x 10^4)."""
+    """The C60 Lennard-Jones case the reference runs on its CPU (He, force type
+    2, 3 x 10^4 trajectories) and the published ubiquitin / He configuration
+    (force type 4, 10 x 10^4)."""
     from massccs_amd import synth
     common = {"seed": 2104, "dt": 10.0, "Temp": 298.0}
     if which == "c60":
@@ -79,165 +100,258 @@ def small_cfg(tmp, which):
         "plain FP64, ubiquitin 04_1ubq_charge_4e_min.pqr + He, force type 4, 10x10^4 trajectories"
 
 
-def host_cores():
+def cpu_baseline(P, cfg, budget_s):
+    """oracle/ port at the reference's release flags (never the product)."""
     try:
-        return len(os.sched_getaffinity(0))
-    except Exception:
-        return os.cpu_count() or 1
-
-
-def cpu_flags():
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for ln in fh:
-                if ln.startswith("flags"):
-                    return set(ln.split(":", 1)[1].split())
-    except OSError:
-        pass
-    return set()
-
-
-def cpu_baseline(cfg, budget_s=15.0):
-    """The reference program (oracle/_ref, built from the reference sources with
-    its release flags) on this host's cores, on a bounded sample of the same
-    workload: trajectories/s from its own "CCS time" stamp."""
-    ref_dir = os.path.join(HERE, "oracle", "_ref")
-    v4 = {"avx512f", "avx512bw", "avx512dq", "avx512vl", "avx512cd"} <= cpu_flags()
-    exe = os.path.join(ref_dir, "massccs_ref_v4" if v4 else "massccs_ref")
-    if not os.path.exists(exe):
-        return None
-    cores = host_cores()
-    tmp = tempfile.mkdtemp(prefix="mccs_cpu_")
-    try:
-        shutil.copy(cfg["targetFileName"], tmp)
-
-        def run(n):
-            c = dict(cfg, targetFileName=os.path.basename(cfg["targetFileName"]),
-                     numberProbe=int(n), nIter=1, nthreads=cores)
-            with open(os.path.join(tmp, "in.json"), "w") as fh:
-                json.dump(c, fh)
-            out = subprocess.run([exe, "in.json"], cwd=tmp, capture_output=True, text=True,
-                                 timeout=600).stdout
-            t = float(re.search(r"CCS time: ([0-9.eE+-]+) s", out).group(1))
-            return t
-        n0 = 64 * cores
-        t0 = run(n0)
-        n1 = int(min(100000, max(n0, n0 * budget_s / max(t0, 1e-3))))
-        t1 = run(n1)
-        return {"value": n1 / t1, "unit": "trajectories/s", "cores": cores, "kind": "reference",
-                "sample": "%d trajectories (first %d of the same seeded stream, nIter 1), "
-                          "reference CLI built %s -O3 -ffast-math -fopenmp, %d OpenMP threads, "
-                          "CCS time %.2f s" % (n1, n1, "-march=x86-64-v4" if v4 else
-                                               "-march=x86-64-v3", cores, t1)}
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import cpu_baseline as CB
+        return CB.baseline(P, cfg, budget_s=budget_s, repeats=5)
     except Exception as e:  # the baseline is reported, never required
-        return {"value": None, "unit": "trajectories/s", "cores": cores, "kind": "reference",
-                "sample": "failed: %s" % e}
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+        return {"value": None, "unit": "trajectories/s", "kind": "port", "sample": "failed: %s" % e}
 
 
-def big_target_cfg(which):
-    """In-memory synthetic targets of the large configurations (N2, type 6)."""
-    from massccs_amd import synth
-    if which == "shell1m":
-        s, xyz, q = synth.shell()
-        label = "1,000,000-atom hollow shell r 120-140 A, net +60 e"
-    else:
-        s, xyz, q = synth.globule(n=50000, axes=(60.0, 45.0, 35.0), net_charge=20.0, seed=2104)
-        label = "50,000-atom ellipsoid 60/45/35 A, net +20 e"
-    cfg = {"targetFileName": "memory.xyz", "GasBuffer": "N2", "Short-range cutoff": "yes",
-           "LJ-cutoff": 12.0, "Long-range forces": "yes", "Long-range cutoff": "yes",
-           "Coul-cutoff": 25.0, "polarizability": "yes", "alpha": 1.7403, "seed": 2104,
-           "dt": 10.0, "Temp": 298.0}
-    return cfg, (s, xyz, q), label
-
-
-def run_big(args, which):
-    """Plain FP64 engine on one large target (samples resident in HBM)."""
-    import torch
-    import massccs_amd as M
-    cfg, (s, xyz, q), label = big_target_cfg(which)
-    n = args.traj or (10 ** 6 if which == "shell1m" else 10 ** 5)
-    P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
-    E = M.Engine(P, count_work=True, lanes=args.lanes)
-    u = torch.from_numpy(P.samples(n)).cuda()
-    rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-    stream = torch.cuda.Stream()
-    torch.cuda.set_stream(stream)
-    for _ in range(args.warmup):
-        E.run_device(u.data_ptr(), min(n, 20000), rec.data_ptr(), stream.cuda_stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    k_ev = []
-    for _ in range(args.steps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        E.run_device(u.data_ptr(), n, rec.data_ptr(), stream.cuda_stream)
-        e1.record(stream)
-        k_ev.append((e0, e1))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
-    om, cnt, mean, err = M.tally(r, 1, n)
-    flops = _flops(r)
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in k_ev]))  # mean over the timed launches
-    sur = None
-    if args.surrogate:
-        t0 = time.perf_counter()
-        rep, _, _ = E.build_surrogate()
-        t_build = time.perf_counter() - t0
-        E.use_surrogate(True)
-        E.run_device(u.data_ptr(), min(n, 20000), rec.data_ptr(), stream.cuda_stream)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            E.run_device(u.data_ptr(), n, rec.data_ptr(), stream.cuda_stream)
-        torch.cuda.synchronize()
-        wall_s = time.perf_counter() - t0
-        E.use_surrogate(False)
-        rs = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
-        _, _, mean_s, _ = M.tally(rs, 1, n)
-        sur = {"value": round(n * args.steps / wall_s, 1), "unit": "trajectories/s",
-               "speedup_vs_plain": round(wall / wall_s, 3),
-               "ccs": mean_s, "ccs_error_vs_plain": round(mean_s / mean - 1, 5),
-               "lost": int(((rs["hit"] == 1) & (rs["success"] == 0)).sum()),
-               "build_seconds": round(t_build, 2),
-               "visits_ratio": round(float(rs["visits"].sum()) / max(1.0, float(r["visits"].sum())), 4),
-               "validation": {"u_rel": [rep["val_rmse_u"][k] / rep["val_rms_u"][k] for k in range(2)],
-                              "f_rel": [rep["val_rmse_f"][k] / rep["val_rms_f"][k] for k in range(2)]},
-               "grid_nodes": rep["grid_nodes"]}
-    return {
-        "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "trajectories/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded target and mt19937 samples)",
-        "config": {"workload": "plain FP64, %s + N2, force type 6, %d trajectories" % (label, n),
-                   "n_atoms": P.info.natoms, "trajectories_per_step": n, "probe": "N2",
-                   "force_type": P.info.force_type, "parallelism": "1 GPU"},
-        "force_evals_per_s": round(3 * int(r["steps"].sum()) / (kern_ms * 1e-3), 1),
-        "ccs": {"mean": mean, "lost": int(((r["hit"] == 1) & (r["success"] == 0)).sum())},
-        "roofline": {"bound": "mfma", "achieved": round(flops / (kern_ms * 1e-3) / 1e12, 3),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "traj_kernel<N2, type 6>",
-                     "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
-        "cpu_baseline": None,
-        "surrogate": sur,
-    }
-
-
-def _flops(r):
+def flops_of(r):
+    """Counted FP64 work of a set of records (the trajectories' own work, the
+    tries whose outcome they report)."""
     hits = r["hit"] == 1
     evals = (1 + r["steps"] + np.maximum(r["tries"], 0)) * hits
     return float(W_VISIT * r["visits"].sum() + W_LJ * r["lj"].sum() + W_COUL * r["coul"].sum()
                  + W_DIP * r["dip"].sum() + W_EPI * evals.sum())
 
 
+def paired_error(rs, rp):
+    """CCS of records rs relative to rp on the same samples (one iteration of
+    all trajectories), and the standard error of that difference from the
+    per-trajectory differences (chaotic trajectories make it non-zero even
+    for rounding-level changes of the force)."""
+    den = float(((rp["hit"] == 0) | (rp["success"] == 1)).sum())
+    dens = float(((rs["hit"] == 0) | (rs["success"] == 1)).sum())
+    cp = rp["domega"].sum() / den
+    cs = rs["domega"].sum() / dens
+    se = float(np.std(rs["domega"] - rp["domega"]) * np.sqrt(len(rp)) / den)
+    return cs / cp - 1.0, se / cp, cp, cs
+
+
+class Timer:
+    """HIP events on the stream the kernels are launched on."""
+
+    def __init__(self, stream):
+        import torch
+        self.torch, self.stream, self.ev = torch, stream, []
+
+    def __enter__(self):
+        self.e0 = self.torch.cuda.Event(enable_timing=True)
+        self.e1 = self.torch.cuda.Event(enable_timing=True)
+        self.e0.record(self.stream)
+        return self
+
+    def __exit__(self, *a):
+        self.e1.record(self.stream)
+        self.ev.append((self.e0, self.e1))
+
+    def mean_ms(self):
+        return float(np.mean([a.elapsed_time(b) for a, b in self.ev]))
+
+
+def run_headline(args, target):
+    """Plain vs surrogate+batched on one large N2 target (samples in HBM)."""
+    import torch
+    import massccs_amd as M
+    cfg, (s, xyz, q), label = target()
+    n = args.traj or 100000
+    P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
+    stream = torch.cuda.Stream()  # default priority; every launch and event goes on it
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    u = torch.from_numpy(P.samples(n)).cuda()
+    rec_p = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    rec_s = torch.zeros_like(rec_p)
+    om = torch.zeros(1, dtype=torch.float64, device="cuda")
+    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
+    me = torch.zeros(2, dtype=torch.float64, device="cuda")
+
+    # plain leg
+    E = M.Engine(P, count_work=True)
+
+    def plain_step(timer=None):
+        if timer:
+            with timer:
+                E.run_device(u.data_ptr(), n, rec_p.data_ptr(), sptr)
+        else:
+            E.run_device(u.data_ptr(), n, rec_p.data_ptr(), sptr)
+        E.tally_device(rec_p.data_ptr(), 1, n, om.data_ptr(), cnt.data_ptr(), me.data_ptr(), sptr)
+
+    for _ in range(args.warmup):
+        plain_step()
+    torch.cuda.synchronize()
+    tp = Timer(stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain_step(tp)
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - t0
+    rp = rec_p.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    flops_p = flops_of(rp)
+
+    # surrogate+batched leg: the target as a batch of one, range-split far field
+    B = M.Batch([P], count_work=True, use_graph=True, skin=1.0)
+    t0 = time.perf_counter()
+    rep = B.build_surrogate(shell=SUR_SHELL, blend=SUR_BLEND, grid=SUR_GRID, exact_table=True)[0]
+    build_s = time.perf_counter() - t0
+    B.use_surrogate(True)
+
+    def sur_step(timer=None):
+        if timer:
+            with timer:
+                B.run_device([n], [u.data_ptr()], [rec_s.data_ptr()], sptr)
+        else:
+            B.run_device([n], [u.data_ptr()], [rec_s.data_ptr()], sptr)
+        E.tally_device(rec_s.data_ptr(), 1, n, om.data_ptr(), cnt.data_ptr(), me.data_ptr(), sptr)
+
+    for _ in range(max(1, args.warmup)):  # the first run captures the graph
+        sur_step()
+    torch.cuda.synchronize()
+    ts = Timer(stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sur_step(ts)
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t0
+    ccs_s = me.cpu().numpy()
+    rs = rec_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    flops_s = flops_of(rs)
+    err, se, cp, cs = paired_error(rs, rp)
+    kern_s = ts.mean_ms()
+    value = n * args.steps / wall_s
+    plain_value = n * args.steps / wall_p
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "trajectories/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall_s / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded target and mt19937 samples)",
+        "config": {"workload": "surrogate+batched vs plain FP64, %s + N2, LJ+Coulomb+induced "
+                               "dipole with cutoffs (force type 6), %d trajectories" % (label, n),
+                   "n_atoms": P.info.natoms, "trajectories_per_step": n, "probe": "N2",
+                   "force_type": P.info.force_type, "parallelism": "1 GPU"},
+        "plain": {"value": round(plain_value, 1), "ms_per_step": round(1e3 * wall_p / args.steps, 3),
+                  "kernel_ms": round(tp.mean_ms(), 3), "ccs": cp,
+                  "force_evals_per_s": round(3 * int(rp["steps"].sum()) / (tp.mean_ms() * 1e-3), 1),
+                  "roofline_frac": round(flops_p / (tp.mean_ms() * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "ccs": {"surrogate": cs, "plain": cp, "error_vs_plain": round(err, 5),
+                "paired_standard_error": round(se, 5), "device_tally_mean": float(ccs_s[0]),
+                "lost": {"surrogate": int(((rs["hit"] == 1) & (rs["success"] == 0)).sum()),
+                         "plain": int(((rp["hit"] == 1) & (rp["success"] == 0)).sum())},
+                "within_0.5%": bool(abs(err) <= 0.005)},
+        "surrogate": {"shell_A": SUR_SHELL, "switch_end_A": SUR_SHELL + SUR_BLEND,
+                      "grid_A": SUR_GRID, "table": "exact far field at the nodes",
+                      "grid_nodes": rep["grid_nodes"], "build_s": round(build_s, 2),
+                      "table_rel_err_psi": rep["val_rmse_u"][0] / max(rep["val_rms_u"][0], 1e-300),
+                      "table_rel_err_grad": rep["val_rmse_f"][0] / max(rep["val_rms_f"][0], 1e-300),
+                      "visits_ratio": round(float(rs["visits"].sum()) / max(1.0, float(rp["visits"].sum())), 4),
+                      "note": "value excludes the one-off table build (build_s)"},
+        "force_evals_per_s": round(3 * int(rs["steps"].sum()) / (kern_s * 1e-3), 1),
+        "roofline": {"bound": "fp64-valu", "achieved": round(flops_s / (kern_s * 1e-3) / 1e12, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops_s / (kern_s * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "traj_kernel<N2, type 6, range-split, batched>",
+                     "kernel_ms": round(kern_s, 3), "flops_per_launch": int(flops_s)},
+    }
+    if args.network:
+        E2 = M.Engine(P, count_work=True)
+        t0 = time.perf_counter()
+        rep2 = E2.build_surrogate(shell=SUR_SHELL, blend=SUR_BLEND, grid=SUR_GRID)[0]
+        b2 = time.perf_counter() - t0
+        E2.use_surrogate(True)
+        E2.run_device(u.data_ptr(), n, rec_s.data_ptr(), sptr)
+        torch.cuda.synchronize()
+        rn = rec_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+        e2, se2, _, cn = paired_error(rn, rp)
+        out["network_table"] = {
+            "ccs_error_vs_plain": round(e2, 5), "paired_standard_error": round(se2, 5),
+            "build_s": round(b2, 2), "train_s": round(rep2["seconds_train"], 2),
+            "net_rel_err_psi": rep2["val_rmse_u"][1] / max(rep2["val_rms_u"][1], 1e-300),
+            "net_rel_err_grad": rep2["val_rmse_f"][1] / max(rep2["val_rms_f"][1], 1e-300)}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(P, cfg, budget_s=args.cpu_budget)
+    return out
+
+
+def run_single(args):
+    """Plain engine on the globule / ubiquitin / C60 workloads (secondary)."""
+    import torch
+    import massccs_amd as M
+    tmp = tempfile.mkdtemp(prefix="mccs_bench_")
+    if args.workload == "globule":
+        cfg = globule_cfg(tmp)
+        label = ("plain FP64, 1231-atom globule + N2, LJ+Coulomb+induced dipole with cutoffs "
+                 "(force type 6), 10x10^4 trajectories")
+    else:
+        cfg, label = small_cfg(tmp, args.workload)
+    P = M.Prepared(M.config_from_dict(cfg))
+    n_iter, n_traj = cfg["nIter"], cfg["numberProbe"]
+    n = n_iter * n_traj
+    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    u = torch.from_numpy(P.samples(n)).cuda()
+    rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    omega = torch.zeros(n_iter, dtype=torch.float64, device="cuda")
+    counts = torch.zeros(3 * n_iter, dtype=torch.int64, device="cuda")
+    mean_err = torch.zeros(2, dtype=torch.float64, device="cuda")
+    tm = Timer(stream)
+
+    def step(timer=None):
+        if timer:
+            with timer:
+                E.run_device(u.data_ptr(), n, rec.data_ptr(), sptr)
+        else:
+            E.run_device(u.data_ptr(), n, rec.data_ptr(), sptr)
+        E.tally_device(rec.data_ptr(), n_iter, n_traj, omega.data_ptr(), counts.data_ptr(),
+                       mean_err.data_ptr(), sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(tm)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = tm.mean_ms()
+    r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    me = mean_err.cpu().numpy()
+    flops = flops_of(r)
+    nsite = 3 if cfg["GasBuffer"] in ("N2", "CO2") else 1
+    out = {
+        "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "trajectories/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded target and mt19937 samples)",
+        "config": {"workload": label, "n_atoms": P.info.natoms, "trajectories_per_step": n,
+                   "probe": cfg["GasBuffer"], "force_type": P.info.force_type,
+                   "lanes_per_trajectory": args.lanes or "auto", "parallelism": "1 GPU"},
+        "force_evals_per_s": round(nsite * int(r["steps"].sum()) / (kern_ms * 1e-3), 1),
+        "ccs": {"mean": float(me[0]), "err": float(me[1])},
+        "roofline": {"bound": "fp64-valu", "achieved": round(flops / (kern_ms * 1e-3) / 1e12, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None,
+                     "kernel": "traj_kernel<%s, type %d>" % (cfg["GasBuffer"], P.info.force_type),
+                     "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(P, cfg, budget_s=args.cpu_budget)
+    return out
+
+
 def run_batch(args):
     """The 64-assembly batch: one persistent launch per probe kind, autotuned
-    launch shapes, replayed from a hipGraph."""
+    launch shapes, replayed from a hipGraph; with --surrogate the members'
+    range-split far fields."""
     import massccs_amd as M
     from massccs_amd import synth
     descs = synth.assemblies()
@@ -255,6 +369,12 @@ def run_batch(args):
         P.append(p)
         us.append(p.samples(per))
     B = M.Batch(P)
+    build = None
+    if args.surrogate:
+        t0 = time.perf_counter()
+        B.build_surrogate(shell=SUR_SHELL, blend=SUR_BLEND, grid=SUR_GRID, exact_table=True)
+        build = time.perf_counter() - t0
+        B.use_surrogate(True)
     t0 = time.perf_counter()
     tune = B.autotune([u[: max(64, per // 16)] for u in us])
     t_tune = time.perf_counter() - t0
@@ -266,25 +386,24 @@ def run_batch(args):
     wall = time.perf_counter() - t0
     n = per * len(P)
     r = np.concatenate(recs)
-    steps = np.array([int(x["steps"].sum()) for x in recs])
     return {
         "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "trajectories/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (64 seeded assemblies and mt19937 samples)",
-        "config": {"workload": "batched FP64, 64 assemblies of 514-199,788 atoms (He type 4 / "
-                               "N2 type 6 alternating), %d trajectories each" % per,
+        "config": {"workload": "batched FP64%s, 64 assemblies of 514-199,788 atoms (He type 4 / "
+                               "N2 type 6 alternating), %d trajectories each"
+                               % (" with range-split far fields" if args.surrogate else "", per),
                    "targets": len(P), "atoms_total": int(sum(p.info.natoms for p in P)),
                    "trajectories_per_step": n, "parallelism": "1 GPU"},
-        "force_evals_per_s": None,
         "steps_per_trajectory": {"min": int(r["steps"][r["hit"] == 1].min()),
                                  "max": int(r["steps"].max()),
                                  "mean": float(r["steps"][r["hit"] == 1].mean())},
         "autotune": {"seconds": round(t_tune, 2),
-                     "choices": [{"class": c["class"].split("/")[0] + "/" + c["class"].split("/")[1],
-                                  "lanes": c["lanes"], "skin": c["skin"]} for c in tune]},
-        "flops": _flops(r), "roofline": None, "cpu_baseline": None,
+                     "choices": [{"class": c["class"], "lanes": c["lanes"], "skin": c["skin"]}
+                                 for c in tune]},
+        "surrogate_build_s": build, "flops": flops_of(r), "roofline": None, "cpu_baseline": None,
     }
 
 
@@ -295,14 +414,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per trajectory: 0 auto, 16, 32 or 64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="globule",
-                    choices=["globule", "batch64", "ellipsoid50k", "shell1m", "c60", "ubq_he"],
-                    help="globule: the measured configuration (1,231 atoms, N2, 10^5 "
-                         "trajectories); the others are the batch and large-target runs")
+    ap.add_argument("--cpu-budget", type=float, default=8.0,
+                    help="seconds of CPU work per baseline run (5 runs, median)")
+    ap.add_argument("--workload", default="ellipsoid50k",
+                    choices=["ellipsoid50k", "shell1m", "globule", "batch64", "c60", "ubq_he"],
+                    help="ellipsoid50k: the headline configuration (plain vs surrogate+batched); "
+                         "the others are secondary measurements")
     ap.add_argument("--traj", type=int, default=0, help="trajectories (per target for batch64)")
-    ap.add_argument("--surrogate", action="store_true",
-                    help="ellipsoid50k/shell1m: also build the far-field network and time the "
-                         "surrogate path against the plain kernel on the same samples")
+    ap.add_argument("--network", action="store_true",
+                    help="headline workloads: also build the far field from the trained network "
+                         "and report its CCS error")
+    ap.add_argument("--surrogate", action="store_true", help="batch64: range-split far fields")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB float32 tensor three times: the calibration of "
                          "the FETCH_SIZE / WRITE_SIZE counters for the traffic figure")
@@ -318,98 +440,15 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the trajectory engine is measured on a single "
-              "MI355X; trajectories would shard by index range across GPUs, which is not run here", file=sys.stderr)
+              "MI355X; trajectories would shard by index range across GPUs, which is not run here",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch64":
-        print(json.dumps(run_batch(args)))
-        return
-    if args.workload in ("ellipsoid50k", "shell1m"):
-        print(json.dumps(run_big(args, args.workload)))
-        return
-
-    import torch
-    import massccs_amd as M
-
-    tmp = tempfile.mkdtemp(prefix="mccs_bench_")
-    if args.workload == "globule":
-        cfg = workload_cfg(tmp)
-        label = ("plain FP64, 1231-atom globule + N2, LJ+Coulomb+induced dipole with cutoffs "
-                 "(force type 6), 10x10^4 trajectories")
+        out = run_batch(args)
+    elif args.workload in ("ellipsoid50k", "shell1m"):
+        out = run_headline(args, ellipsoid50k if args.workload == "ellipsoid50k" else shell1m)
     else:
-        cfg, label = small_cfg(tmp, args.workload)
-    P = M.Prepared(M.config_from_dict(cfg))
-    info = P.info
-    n_iter, n_traj = cfg["nIter"], cfg["numberProbe"]
-    n = n_iter * n_traj
-    E = M.Engine(P, device=0, lanes=args.lanes, count_work=True)
-
-    u = torch.from_numpy(P.samples(n)).cuda()
-    rec = torch.zeros((n, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-    omega = torch.zeros(n_iter, dtype=torch.float64, device="cuda")
-    counts = torch.zeros(3 * n_iter, dtype=torch.int64, device="cuda")
-    mean_err = torch.zeros(2, dtype=torch.float64, device="cuda")
-    # a stream of our own (default priority): the kernels and the HIP events
-    # that time them are enqueued on it
-    stream = torch.cuda.Stream()
-    torch.cuda.set_stream(stream)
-    sptr = stream.cuda_stream
-
-    def step():
-        E.run_device(u.data_ptr(), n, rec.data_ptr(), sptr)
-        E.tally_device(rec.data_ptr(), n_iter, n_traj, omega.data_ptr(), counts.data_ptr(),
-                       mean_err.data_ptr(), sptr)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    k_ms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        E.run_device(u.data_ptr(), n, rec.data_ptr(), sptr)
-        e1.record(stream)
-        E.tally_device(rec.data_ptr(), n_iter, n_traj, omega.data_ptr(), counts.data_ptr(),
-                       mean_err.data_ptr(), sptr)
-        k_ms.append((e0, e1))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
-
-    executed = E.last_work()  # every try run, including side-by-side tries left unused
-    r = rec.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
-    me = mean_err.cpu().numpy()
-    flops = _flops(r)
-    achieved = flops / (kern_ms * 1e-3) / 1e12
-    value = n * args.steps / wall
-    nsite = 3 if cfg["GasBuffer"] in ("N2", "CO2") else 1
-    site_evals = nsite * int(r["steps"].sum())
-    out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "trajectories/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded globule target and mt19937 samples)",
-        "config": {"workload": label,
-                   "n_atoms": info.natoms, "trajectories_per_step": n, "probe": cfg["GasBuffer"],
-                   "force_type": info.force_type, "lanes_per_trajectory": args.lanes or "auto",
-                   "parallelism": "1 GPU"},
-        "force_evals_per_s": round(site_evals / (kern_ms * 1e-3), 1),
-        "ccs": {"mean": float(me[0]), "err": float(me[1])},
-        "steps_per_launch": {"of_records": int(r["steps"].sum()),
-                             "executed": int(executed["steps"])},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": TRAFFIC_BYTES if args.workload == "globule" else None,
-                     "kernel": "traj_kernel<%s, type %d>" % (cfg["GasBuffer"], info.force_type),
-                     "kernel_ms": round(kern_ms, 3), "flops_per_launch": int(flops)},
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg)
-    shutil.rmtree(tmp, ignore_errors=True)
+        out = run_single(args)
     print(json.dumps(out))
 
 

@@ -261,6 +261,14 @@ void mccs_batch_destroy(mccs_batch *b);
 mccs_status mccs_batch_run(mccs_batch *b, const uint64_t *n, const double *const *samples,
                            mccs_traj_record *const *records);
 
+/* Same with device-resident samples[t] and records[t] (HBM pointers),
+ * enqueued on a HIP stream (NULL = the batch's own stream) without
+ * synchronising: the batch's launches start after the work already on that
+ * stream and finish before later work on it. */
+mccs_status mccs_batch_run_device(mccs_batch *b, const uint64_t *n,
+                                  const double *const *samples_dev,
+                                  mccs_traj_record *const *records_dev, void *stream);
+
 /* Autotuner: for every group, times trial runs of the first n_trial[t]
  * samples of each target over lanes per trajectory {16, 32, 64} and Verlet
  * skin {0.5, 1, 2} A, keeps the fastest, and caches it for this process by
@@ -279,9 +287,9 @@ mccs_status mccs_batch_last_work(mccs_batch *b, uint64_t totals[5]);
 /* ------------------------------------------------------------------ */
 /* Far-field surrogate (no counterpart in the reference).  A Fourier-feature
  * MLP -- 64 features [sin(2 pi B r'), cos(2 pi B r')], r' = (r - centre) *
- * in_scale, then 3 x 128 SiLU, then 2 outputs (energy of the N2 nitrogen
- * site and of the N2 charge/dipole site, times out_scale) -- trained on
- * energies that the exact linked-cell evaluator computes.  BF16 weights,
+ * in_scale, then 3 x 128 SiLU, then 2 outputs (times out_scale) -- trained on
+ * the far field psi that the exact linked-cell evaluator computes (both
+ * outputs are fitted to psi; the far field uses output 0).  BF16 weights,
  * FP32 accumulation, all products on MFMA.  Parameters are a flat FP32 array
  * of MCCS_MLP_NPARAM values: W1[64][128], b1[128], W2[128][128], b2[128],
  * W3[128][128], b3[128], W4[128][2], b4[2] (row-major, W[in][out]). */
@@ -312,39 +320,65 @@ mccs_status mccs_mlp_train(int32_t device, const mccs_mlp_spec *spec, float *par
                            int32_t batch, double lr, uint32_t seed, double *loss_history);
 
 typedef struct {
-  uint64_t n_train;        /* points sampled by the exact evaluator (default 2e6) */
+  uint64_t n_train;        /* points at which the exact far field is sampled to train the
+                              network (default 2e6; table_source 0 only) */
   int32_t epochs, batch;   /* default 300, 8192 */
   double lr;               /* default 5e-3 (cosine decay to lr / 20) */
-  double shell;            /* exact force within this distance of any atom (default 8 A) */
-  double blend;            /* exact and surrogate blended over [shell, shell + blend] (2 A) */
-  double grid;             /* spacing of the tabulated network (default 0.25 A) */
+  double shell;            /* pairs nearer than this keep the exact terms (default 8 A) */
+  double blend;            /* width of the switch to the far field (default 4 A): pair terms
+                              reach zero at shell + blend, which must not exceed the Coulomb
+                              cutoff */
+  double grid;             /* spacing of the tabulated far field (default 0.5 A) */
   double feature_length;   /* Fourier frequencies ~ N(0, 1 / feature_length^2) per A; 0 = auto,
-                              0.23 x (largest target extent + 2 x the longest cutoff) */
+                              0.23 x (largest target extent + 2 x the Coulomb cutoff) */
   uint32_t seed;
-  int32_t table_source;    /* 0: the trained network; 1: the exact evaluator at the grid nodes
-                              (no network; validates the tabulated path) */
+  int32_t table_source;    /* 0: the trained network is tabulated; 1: the exact far field is
+                              evaluated at the grid nodes (no network); 2: no far field (the
+                              near terms alone: the ablation) */
 } mccs_surrogate_opts;
 
 typedef struct {
-  double train_loss;                 /* last epoch, scaled units */
-  double val_rmse_u[2], val_rms_u[2];  /* held-out energy error and scale per output, kcal/mol */
-  double val_rmse_f[2], val_rms_f[2];  /* held-out force error and scale per output, kcal/mol/A */
+  double train_loss;                 /* last epoch, scaled units (network only) */
+  /* held-out error and scale of psi (value, [0] and [1]) and of grad psi
+   * (per component, 1/A): [0] the tabulated field the trajectories use,
+   * [1] the network before tabulation (zero without a network); against the
+   * exact far field at points 2 A or more from every atom */
+  double val_rmse_u[2], val_rms_u[2];
+  double val_rmse_f[2], val_rms_f[2];
   double seconds_sample, seconds_train, seconds_table;
   uint64_t n_train, grid_nodes;
   mccs_mlp_spec spec;
 } mccs_surrogate_report;
 
-/* Builds the far-field surrogate of an N2 (force type 6) engine's target on
- * its GPU: samples points outside the shell, evaluates them exactly, trains
- * the network with the MFMA kernels, and tabulates it on a cubic B-spline
- * grid that the trajectory kernel interpolates.  params_out (may be NULL)
- * receives the trained parameters. */
+/* Range-split far field of an engine whose probe carries an induced dipole
+ * (N2 force type 6, He/Ar/co2 force type 4); no counterpart in the
+ * reference.  Every electrostatic pair term of src/Force.cpp (the shifted
+ * Coulomb potential and the smoothed ion-induced-dipole field) is split by a
+ * smooth partition S(r) of the pair distance: S = 1 below `shell`, 0 beyond
+ * shell + blend (quintic smoothstep between).  The trajectory kernel keeps
+ * summing S times the exact terms over the atoms within shell + blend (with
+ * the Lennard-Jones terms, whose cutoff the list covers); the remainder is
+ * one smooth scalar field psi(x) = sum_j q_j (1 - S) u_c(r_j),
+ * u_c = 1/r - 1.5/rc + r^2/(2 rc^3), whose value, gradient and Hessian give
+ * every far-field term: a charged site's q K psi and -q K grad psi, the
+ * dipole field's E = -grad psi and grad E = -Hess psi.  psi is tabulated as
+ * interpolating cubic B-spline coefficients: from the Fourier-feature MLP
+ * trained (BF16 MFMA) on psi sampled by the linked-cell evaluator
+ * (table_source 0), or from that evaluator at every node (table_source 1).
+ * params_out (may be NULL) receives the trained parameters. */
 mccs_status mccs_engine_build_surrogate(mccs_engine *e, const mccs_surrogate_opts *opts,
                                         mccs_surrogate_report *report, float *params_out);
 
-/* 1: later runs use the surrogate outside the shell (exact force inside it,
- * blended across the blend band); 0: the exact force everywhere. */
+/* 1: later runs use the range-split far field (64 lanes per trajectory);
+ * 0: the exact force everywhere. */
 mccs_status mccs_engine_use_surrogate(mccs_engine *e, int32_t on);
+
+/* The same for every target of a batch that supports it (the others stay
+ * exact); reports (may be NULL) receives one report per target (zero for
+ * the others). */
+mccs_status mccs_batch_build_surrogate(mccs_batch *b, const mccs_surrogate_opts *opts,
+                                       mccs_surrogate_report *reports);
+mccs_status mccs_batch_use_surrogate(mccs_batch *b, int32_t on);
 
 /* Library version string. */
 const char *mccs_version(void);

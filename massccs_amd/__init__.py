@@ -105,6 +105,7 @@ EXPORTS = [
     "mccs_batch_destroy", "mccs_batch_run", "mccs_batch_autotune", "mccs_batch_shapes",
     "mccs_batch_last_work", "mccs_mlp_eval", "mccs_mlp_loss_grad", "mccs_mlp_train",
     "mccs_engine_build_surrogate", "mccs_engine_use_surrogate", "mccs_run_device_noalloc",
+    "mccs_batch_build_surrogate", "mccs_batch_use_surrogate", "mccs_batch_run_device",
 ]
 
 _lib = None
@@ -166,6 +167,11 @@ def lib():
     L.mccs_engine_build_surrogate.argtypes = [C.c_void_p, C.POINTER(SurrogateOpts),
                                               C.POINTER(SurrogateReport), fp]
     L.mccs_engine_use_surrogate.argtypes = [C.c_void_p, C.c_int32]
+    L.mccs_batch_build_surrogate.argtypes = [C.c_void_p, C.POINTER(SurrogateOpts),
+                                             C.POINTER(SurrogateReport)]
+    L.mccs_batch_use_surrogate.argtypes = [C.c_void_p, C.c_int32]
+    L.mccs_batch_run_device.argtypes = [C.c_void_p, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]
     L.mccs_tally.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                              C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_tally_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
@@ -335,12 +341,17 @@ class Engine:
         return dict(zip(("visits", "lj", "coul", "dip", "steps"), list(out)))
 
     def build_surrogate(self, n_train=2_000_000, epochs=300, batch=8192, lr=5e-3, shell=8.0,
-                        blend=2.0, grid=0.25, feature_length=0.0, seed=2104, exact_table=False):
-        """Trains and tabulates the far-field network of this (N2, type 6)
-        engine's target; returns (report dict, trained parameters, spec).
-        exact_table tabulates the exact evaluator instead of the network."""
-        o = SurrogateOpts(n_train, epochs, batch, lr, shell, blend, grid, feature_length, seed,
-                          1 if exact_table else 0)
+                        blend=4.0, grid=0.5, feature_length=0.0, seed=2104, exact_table=False,
+                        table_source=None):
+        """Range-split far field of this engine's target (N2 type 6, He/Ar/co2
+        type 4): pairs nearer than shell keep the exact terms, the switch
+        reaches zero at shell + blend, and the remainder psi is tabulated on a
+        cubic B-spline grid -- from the network trained on the exact far field
+        (default) or, with exact_table, from the exact field at the nodes
+        (table_source 2: no far field, the ablation).
+        Returns (report dict, trained parameters, spec)."""
+        src = table_source if table_source is not None else (1 if exact_table else 0)
+        o = SurrogateOpts(n_train, epochs, batch, lr, shell, blend, grid, feature_length, seed, src)
         r = SurrogateReport()
         params = np.zeros(MLP_NPARAM, dtype=np.float32)
         _check(lib().mccs_engine_build_surrogate(self.h, C.byref(o), C.byref(r),
@@ -396,6 +407,17 @@ class Batch:
         _check(lib().mccs_batch_run(self.h, n, up, rp))
         return recs
 
+    def run_device(self, n, sample_ptrs, record_ptrs, stream=None):
+        """Device-resident run: n[t] trajectories of target t from the device
+        samples at sample_ptrs[t] into the device records at record_ptrs[t],
+        enqueued on stream (a HIP stream handle, None = the batch's own)."""
+        k = len(self.targets)
+        nn = (C.c_uint64 * k)(*[int(x) for x in n])
+        sp = (C.c_void_p * k)(*sample_ptrs)
+        rp = (C.c_void_p * k)(*record_ptrs)
+        _check(lib().mccs_batch_run_device(self.h, nn, sp, rp,
+                                           C.c_void_p(stream) if stream else None))
+
     def autotune(self, samples):
         """Trial runs over the given samples (a prefix of each target's);
         returns the tuner's report (JSON)."""
@@ -405,6 +427,20 @@ class Batch:
         buf = C.create_string_buffer(1 << 20)
         _check(lib().mccs_batch_autotune(self.h, n, up, buf, len(buf), C.byref(ln)))
         return json.loads(buf.value.decode())
+
+    def build_surrogate(self, n_train=2_000_000, epochs=300, batch=8192, lr=5e-3, shell=8.0,
+                        blend=4.0, grid=0.5, feature_length=0.0, seed=2104, exact_table=False):
+        """Far fields of every member that carries a dipole field (see
+        Engine.build_surrogate); returns one report dict per target."""
+        o = SurrogateOpts(n_train, epochs, batch, lr, shell, blend, grid, feature_length, seed,
+                          1 if exact_table else 0)
+        reps = (SurrogateReport * len(self.targets))()
+        _check(lib().mccs_batch_build_surrogate(self.h, C.byref(o), reps))
+        return [{k: (list(getattr(r, k)) if k.startswith("val_") else getattr(r, k))
+                 for k, _ in SurrogateReport._fields_ if k != "spec"} for r in reps]
+
+    def use_surrogate(self, on=True):
+        _check(lib().mccs_batch_use_surrogate(self.h, 1 if on else 0))
 
     def shapes(self):
         k = len(self.targets)

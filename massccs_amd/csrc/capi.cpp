@@ -354,6 +354,32 @@ mccs_status mccs_engine_use_surrogate(mccs_engine *e, int32_t on) {
   });
 }
 
+mccs_status mccs_batch_run_device(mccs_batch *b, const uint64_t *n,
+                                  const double *const *samples_dev,
+                                  mccs_traj_record *const *records_dev, void *stream) {
+  return guard([&] {
+    need(b && n && samples_dev && records_dev, "null argument");
+    mccs::batch_run_device(b->b, n, samples_dev, records_dev, stream);
+  });
+}
+
+mccs_status mccs_batch_build_surrogate(mccs_batch *b, const mccs_surrogate_opts *opts,
+                                       mccs_surrogate_report *reports) {
+  return guard([&] {
+    need(b, "null batch");
+    mccs_surrogate_opts o{};
+    if (opts) o = *opts;
+    mccs::batch_build_surrogate(b->b, o, reports);
+  });
+}
+
+mccs_status mccs_batch_use_surrogate(mccs_batch *b, int32_t on) {
+  return guard([&] {
+    need(b, "null batch");
+    mccs::batch_use_surrogate(b->b, on != 0);
+  });
+}
+
 // src/System.cpp:287-317
 mccs_status mccs_tally(const mccs_traj_record *rec, uint64_t n_iter, uint64_t n_traj,
                        double *omega_iter, int64_t *counts, double *ccs_mean, double *ccs_err) {

@@ -17,11 +17,11 @@ namespace mccs {
 namespace dev {
 
 // Per-point site forces: one group of L lanes per point.
-template <class S, int L, bool LDS>
+template <class S, int L, bool LDS, bool SPLIT = false>
 __global__ __launch_bounds__(512) void eval_kernel(Tgt T, const double *__restrict__ xyz, uint64_t n,
                                                    double qsite, double *__restrict__ fout,
                                                    double *__restrict__ uout) {
-  extern __shared__ __align__(16) char smem[];
+  extern __shared__ __align__(32) char smem[];
   Shm sh;
   stage<L, LDS, any_central<S>()>(T, sh, smem);
   const int lane = threadIdx.x % L;
@@ -31,7 +31,7 @@ __global__ __launch_bounds__(512) void eval_kernel(Tgt T, const double *__restri
     double p[1][3] = {{xyz[3 * g], xyz[3 * g + 1], xyz[3 * g + 2]}};
     double qs[1] = {qsite};
     double f[1][3], U[1];
-    evaluate<S, L, LDS, false>(T, sh, lane, p, qs, f, U, w, nullptr);
+    evaluate<S, L, LDS, false, SPLIT>(T, sh, lane, p, qs, f, U, w, nullptr);
     if (lane == 0) {
       fout[3 * g] = f[0][0];
       fout[3 * g + 1] = f[0][1];
@@ -82,11 +82,11 @@ __global__ void tally_final_kernel(const double *__restrict__ omega, uint64_t n_
 // ---------------------------------------------------------------- dispatch
 using EvalFn = void (*)(Tgt, const double *, uint64_t, double, double *, double *);
 
-TrajFn select_traj(int gk, int ft, int L, bool lds, bool count) {
+TrajFn select_traj(int gk, int ft, int L, bool lds, bool count, bool split) {
   switch (gk) {
-    case 1: return select_traj_gk1(ft, L, lds, count);
-    case 2: return select_traj_gk2(ft, L, lds, count);
-    case 3: return select_traj_gk3(ft, L, lds, count);
+    case 1: return select_traj_gk1(ft, L, lds, count, split);
+    case 2: return select_traj_gk2(ft, L, lds, count, split);
+    case 3: return split ? nullptr : select_traj_gk3(ft, L, lds, count);
   }
   return nullptr;
 }
@@ -99,6 +99,21 @@ EvalFn pick_eval(bool lds) {
 template <bool LJ, bool CEN, bool COUL, bool DIP>
 EvalFn pick_eval_c(bool cut, bool lds) {
   return cut ? pick_eval<LJ, CEN, COUL, DIP, true>(lds) : pick_eval<LJ, CEN, COUL, DIP, false>(lds);
+}
+
+// The site roles whose terms the range-split far field covers (N2 nitrogen
+// and charge/dipole sites, the one-site dipole probe), with cutoffs.
+EvalFn select_eval_split(bool lj, bool coul, bool dip, bool lds) {
+  if (lj && coul && !dip)
+    return lds ? eval_kernel<OneSite<true, false, true, false, true>, 64, true, true>
+               : eval_kernel<OneSite<true, false, true, false, true>, 64, false, true>;
+  if (!lj && coul && dip)
+    return lds ? eval_kernel<OneSite<false, false, true, true, true>, 64, true, true>
+               : eval_kernel<OneSite<false, false, true, true, true>, 64, false, true>;
+  if (lj && !coul && dip)
+    return lds ? eval_kernel<OneSite<true, false, false, true, true>, 64, true, true>
+               : eval_kernel<OneSite<true, false, false, true, true>, 64, false, true>;
+  return nullptr;
 }
 
 EvalFn select_eval(bool lj, bool cen, bool coul, bool dip, bool cut, bool lds) {
@@ -300,6 +315,7 @@ struct Engine {
   int gk = 1, ft = 2, lanes = 64, block = 256, cus = 0;
   bool lds = false, count = false;
   size_t shmem = 0;
+  size_t shmem_split = 0;  // LDS per block of the range-split kernel
   Tgt T{};
   double site_q[5] = {0, -0.4825, 0.965, -0.3256, 0.6512};
   std::vector<void *> allocs;
@@ -307,8 +323,10 @@ struct Engine {
   Tgt *d_tg = nullptr;  // T in device memory (the kernel's target array)
   // far-field surrogate
   Mlp *mlp = nullptr;
-  float4 *d_sg = nullptr;
+  float *d_psi = nullptr;   // B-spline coefficients of the far field
+  Tgt Ts{};                 // T with the range-split far field
   bool sur_built = false;
+  bool split = false;       // runs use Ts
   int spec_below = -1;
   hipStream_t last_stream = nullptr;
   const unsigned long long *last_ctl = nullptr;  // queue counters of the last run
@@ -330,7 +348,7 @@ struct Engine {
   ~Engine() {
     if (device >= 0) hipSetDevice(device);
     if (mlp) mlp_destroy(mlp);
-    if (d_sg) hipFree(d_sg);
+    if (d_psi) hipFree(d_psi);
     for (void *p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
   }
@@ -497,20 +515,22 @@ struct TrajLaunch {
   TrajFn fn = nullptr;
   uint64_t blocks = 0;
   long long spec_below = 0;
+  size_t shmem = 0;
 };
 
 static TrajLaunch traj_launch(Engine *E, uint64_t n) {
   TrajLaunch tl;
-  tl.fn = select_traj(E->gk, E->ft, E->lanes, E->lds, E->count);
-  if (!tl.fn) {  // force types without tuned variants run 64 lanes with work counters
+  tl.fn = select_traj(E->gk, E->ft, E->lanes, E->lds, E->count, E->split);
+  if (!tl.fn && !E->split) {  // force types without tuned variants run 64 lanes with work counters
     E->lanes = 64;
     E->count = true;
-    tl.fn = select_traj(E->gk, E->ft, 64, E->lds, true);
+    tl.fn = select_traj(E->gk, E->ft, 64, E->lds, true, false);
   }
   if (!tl.fn) throw Error(MCCS_ERR_UNSUPPORTED, "no trajectory kernel for this gas / force type");
-  set_shmem((const void *)tl.fn, E->shmem);
+  tl.shmem = E->split ? E->shmem_split : E->shmem;
+  set_shmem((const void *)tl.fn, tl.shmem);
   int per_cu = 0;
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)tl.fn, E->block, E->shmem));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)tl.fn, E->block, tl.shmem));
   if (per_cu < 1) throw Error(MCCS_ERR_DEVICE, "trajectory kernel does not fit on a CU");
   const uint64_t groups_per_block = E->block / E->lanes;
   tl.blocks = std::min<uint64_t>((uint64_t)per_cu * E->cus,
@@ -527,7 +547,7 @@ static void launch_traj(Engine *E, const TrajLaunch &tl, Queue Q, const double *
   int ntg = 1;
   void *args[] = {&tg, &toff, &ntg, &u_dev, &n, &rec_dev, &Q};
   HIP_OK(hipLaunchKernel((const void *)tl.fn, dim3((unsigned)tl.blocks), dim3(E->block), args,
-                         E->shmem, s));
+                         tl.shmem, s));
   E->last_stream = s;
 }
 
@@ -660,7 +680,8 @@ static EvalFn site_kernel(const Engine *E, int site, double &q) {
     throw Error(MCCS_ERR_INVALID, "unknown site role");
   }
   if (!(lj || coul || dip)) return nullptr;  // N2 dummy of force types 1/2: no interaction
-  EvalFn fn = select_eval(lj, cen, coul, dip, cut, E->lds);
+  EvalFn fn = E->split ? select_eval_split(lj, coul, dip, E->lds)
+                       : select_eval(lj, cen, coul, dip, cut, E->lds);
   if (!fn) throw Error(MCCS_ERR_UNSUPPORTED, "no site kernel for this role");
   return fn;
 }
@@ -679,8 +700,7 @@ static void eval_sites_dev(Engine *E, int site, const double *dx, uint64_t n, do
   set_shmem((const void *)fn, shmem);
   const uint64_t gpb = block / 64;
   const uint64_t blocks = std::min<uint64_t>((n + gpb - 1) / gpb, (uint64_t)E->cus * 8);
-  Tgt T = E->T;
-  T.sg = nullptr;
+  Tgt T = E->split ? E->Ts : E->T;  // with the far field when it is in use
   void *args[] = {&T, &dx, &n, &q, &df, &du};
   HIP_OK(hipLaunchKernel((const void *)fn, dim3((unsigned)blocks), dim3(block), args, shmem, E->stream));
 }
@@ -723,7 +743,166 @@ void engine_tally_device(Engine *E, const mccs_traj_record *rec_dev, uint64_t n_
 
 
 // ---------------------------------------------------------------- surrogate
+// Range-split far field (see Tgt): psi(x) = sum_j q_j (1 - S(r_j)) u_c(r_j)
+// over the atoms within the Coulomb cutoff, its exact evaluation through the
+// linked cells, the network trained on it, and its tabulation as cubic
+// B-spline coefficients.
 namespace dev {
+
+struct FarGrid {
+  double ox, oy, oz, h;
+  int nx, ny, nz;
+};
+
+// Partition of unity of the pair distance: 1 below sw1, 0 beyond sw2.
+__device__ __forceinline__ void switch_fn(const Tgt &T, double r, double &S, double &dS) {
+  const double t = fmin(fmax((r - T.sw1) * T.isw, 0.0), 1.0), t2 = t * t, u = 1.0 - t;
+  S = 1.0 - t2 * t * fma(t, fma(6.0, t, -15.0), 10.0);
+  dS = -30.0 * t2 * u * u * T.isw;
+}
+
+// psi and grad psi at n points (or at the nodes of a grid when g is given):
+// one wavefront per point, the atoms of the cell columns within the cutoff
+// strided over its lanes.  grad may be null.
+template <bool GRID>
+__global__ __launch_bounds__(256) void far_eval_kernel(Tgt T, const double *__restrict__ xyz,
+                                                      FarGrid g, uint64_t n,
+                                                      double *__restrict__ val,
+                                                      double *__restrict__ grad) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t pi = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; pi < n;
+       pi += waves) {
+    double x, y, z;
+    if (GRID) {
+      const uint64_t iz = pi % g.nz, iy = (pi / g.nz) % g.ny, ix = pi / ((uint64_t)g.ny * g.nz);
+      x = g.ox + g.h * ix;
+      y = g.oy + g.h * iy;
+      z = g.oz + g.h * iz;
+    } else {
+      x = xyz[3 * pi];
+      y = xyz[3 * pi + 1];
+      z = xyz[3 * pi + 2];
+    }
+    const double R = T.rc;
+    const int ix0 = clampi((int)floor((x - R - T.cx) * T.inv_cell), 0, T.nx - 1);
+    const int ix1 = clampi((int)floor((x + R - T.cx) * T.inv_cell), 0, T.nx - 1);
+    const int iy0 = clampi((int)floor((y - R - T.cy) * T.inv_cell), 0, T.ny - 1);
+    const int iy1 = clampi((int)floor((y + R - T.cy) * T.inv_cell), 0, T.ny - 1);
+    double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    // nothing within the cutoff of the target's box: psi = 0
+    double d2 = 0.0;
+    const double pp[3] = {x, y, z};
+    for (int d = 0; d < 3; d++) {
+      const double e = fmax(0.0, fmax(T.blo[d] - pp[d], pp[d] - T.bhi[d]));
+      d2 += e * e;
+    }
+    if (d2 < T.rfar2) {
+      for (int i = ix0; i <= ix1; i++)
+        for (int j = iy0; j <= iy1; j++) {
+          const double lox = T.cx + i * T.cell, loy = T.cy + j * T.cell;
+          const double dxm = fmax(0.0, fmax(lox - x, x - (lox + T.cell)));
+          const double dym = fmax(0.0, fmax(loy - y, y - (loy + T.cell)));
+          const double c2 = dxm * dxm + dym * dym;
+          if (c2 >= R * R) continue;
+          const double hh = sqrt(R * R - c2);
+          const int k0 = clampi((int)floor((z - hh - T.cz) * T.inv_cell), 0, T.nz - 1);
+          const int k1 = clampi((int)floor((z + hh - T.cz) * T.inv_cell), 0, T.nz - 1);
+          const int b = T.nz * (j + T.ny * i);
+          const int a0 = T.cstart[b + k0], a1 = T.cstart[b + k1 + 1];
+          for (int a = a0 + lane; a < a1; a += 64) {
+            const double4 at = T.atoms[a];
+            const double dx = x - at.x, dy = y - at.y, dz = z - at.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < T.rc2cut) || r2 == 0.0) continue;
+            const double r = sqrt(r2), rinv = 1.0 / r;
+            double S, dS;
+            switch_fn(T, r, S, dS);
+            const double uc = rinv + T.cc1 + T.cc2 * r2;
+            const double u1 = r * T.irc3 - rinv * rinv;
+            // (1 - S) u and its radial derivative
+            const double ph = (1.0 - S) * uc, ph1 = (1.0 - S) * u1 - dS * uc;
+            v += at.w * ph;
+            const double gr = at.w * ph1 * rinv;
+            gx += gr * dx;
+            gy += gr * dy;
+            gz += gr * dz;
+          }
+        }
+    }
+    v = gsum<64>(v);
+    gx = gsum<64>(gx);
+    gy = gsum<64>(gy);
+    gz = gsum<64>(gz);
+    if (lane == 0) {
+      val[pi] = v;
+      if (grad) {
+        grad[3 * pi] = gx;
+        grad[3 * pi + 1] = gy;
+        grad[3 * pi + 2] = gz;
+      }
+    }
+  }
+}
+
+// Interpolating cubic B-spline coefficients from node values, one axis at a
+// time (in place): the causal and anti-causal recursions of the pole
+// z1 = sqrt(3) - 2 (Unser's direct B-spline filter) with the field taken as
+// zero beyond the grid, where it vanishes.  One thread per grid line.
+__global__ void bspline_prefilter_kernel(double *__restrict__ c, int n, uint64_t nlines,
+                                         uint64_t stride, uint64_t inner) {
+  const uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlines) return;
+  // line l: its first element; lines enumerate (outer, inner) index pairs
+  const uint64_t base = (l / inner) * inner * (uint64_t)n + (l % inner);
+  const double z1 = 1.7320508075688772 - 2.0;
+  double *p = c + base;
+  double cp = p[0];
+  for (int k = 1; k < n; k++) {
+    cp = p[(uint64_t)k * stride] + z1 * cp;
+    p[(uint64_t)k * stride] = cp;
+  }
+  double cm = -z1 / (1.0 - z1 * z1) * p[(uint64_t)(n - 1) * stride];
+  p[(uint64_t)(n - 1) * stride] = 6.0 * cm;
+  for (int k = n - 2; k >= 0; k--) {
+    cm = z1 * (cm - p[(uint64_t)k * stride]);
+    p[(uint64_t)k * stride] = 6.0 * cm;
+  }
+}
+
+// psi and grad psi of the tabulated spline at n points (validation).
+__global__ void spline_eval_kernel(Tgt T, const double *__restrict__ xyz, uint64_t n,
+                                   double *__restrict__ val, double *__restrict__ grad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  const double u[3] = {(p[0] - T.pox) * T.pih, (p[1] - T.poy) * T.pih, (p[2] - T.poz) * T.pih};
+  long i0[3];
+  double t[3];
+  for (int d = 0; d < 3; d++) {
+    const double f = floor(u[d]);
+    i0[d] = (long)f - 1;
+    t[d] = u[d] - f;
+  }
+  double v = 0, g[3] = {0, 0, 0};
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++)
+      for (int cc = 0; cc < 4; cc++) {
+        const long ix = i0[0] + a, iy = i0[1] + b, iz = i0[2] + cc;
+        if (ix < 0 || ix >= T.pnx || iy < 0 || iy >= T.pny || iz < 0 || iz >= T.pnz) continue;
+        const double c = T.psi[((size_t)ix * T.pny + iy) * T.pnz + iz];
+        double wx, dwx, ddx, wy, dwy, ddy, wz, dwz, ddz;
+        bspline_piece(a, t[0], wx, dwx, ddx);
+        bspline_piece(b, t[1], wy, dwy, ddy);
+        bspline_piece(cc, t[2], wz, dwz, ddz);
+        v += c * wx * wy * wz;
+        g[0] += c * dwx * wy * wz * T.pih;
+        g[1] += c * wx * dwy * wz * T.pih;
+        g[2] += c * wx * wy * dwz * T.pih;
+      }
+  val[i] = v;
+  for (int d = 0; d < 3; d++) grad[3 * i + d] = g[d];
+}
 
 // Distance from each point to the nearest target atom, capped at cap
 // (cells within ceil(cap / cell) of the point's cell).
@@ -750,21 +929,20 @@ __global__ void mindist_kernel(Tgt T, const float *__restrict__ xyz, uint64_t n,
 }
 
 // Positions of grid nodes [first, first + n) (x-major).
-__global__ void grid_points_kernel(uint64_t first, uint64_t n, double ox, double oy, double oz,
-                                   double h, int ny, int nz, float *__restrict__ out) {
+__global__ void grid_points_kernel(uint64_t first, uint64_t n, FarGrid g, float *__restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t g = first + i;
-  const uint64_t ix = g / ((uint64_t)ny * nz), iy = (g / nz) % ny, iz = g % nz;
-  out[3 * i] = (float)(ox + h * ix);
-  out[3 * i + 1] = (float)(oy + h * iy);
-  out[3 * i + 2] = (float)(oz + h * iz);
+  const uint64_t q = first + i;
+  const uint64_t ix = q / ((uint64_t)g.ny * g.nz), iy = (q / g.nz) % g.ny, iz = q % g.nz;
+  out[3 * i] = (float)(g.ox + g.h * ix);
+  out[3 * i + 1] = (float)(g.oy + g.h * iy);
+  out[3 * i + 2] = (float)(g.oz + g.h * iz);
 }
 
-// Grid node: both energies, tapered to zero between ts and te (distance to
-// the nearest atom), and the distance capped at dcap.
-__global__ void pack_grid_kernel(const float *__restrict__ y, const float *__restrict__ d, uint64_t n,
-                                 float ts, float te, float dcap, float4 *__restrict__ out) {
+// Network output 0 at grid nodes, in psi units, tapered to zero between ts
+// and te (distance to the nearest atom) where the exact field reaches zero.
+__global__ void pack_network_kernel(const float *__restrict__ y, const float *__restrict__ d,
+                                    uint64_t n, float ts, float te, double *__restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float di = d[i];
@@ -774,20 +952,7 @@ __global__ void pack_grid_kernel(const float *__restrict__ y, const float *__res
     const float x = (di - ts) / (te - ts);
     t = 1.f - x * x * (3.f - 2.f * x);
   }
-  out[i] = make_float4(y[2 * i] * t, y[2 * i + 1] * t, fminf(di, dcap), 0.f);
-}
-
-__global__ void f32_to_f64_kernel(const float *__restrict__ a, uint64_t n, double *__restrict__ b) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (double)a[i];
-}
-
-__global__ void pair_to_f32_kernel(const double *__restrict__ u0, const double *__restrict__ u1,
-                                   uint64_t n, float *__restrict__ y) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  y[2 * i] = (float)u0[i];
-  y[2 * i + 1] = (float)u1[i];
+  out[i] = (double)(y[2 * i] * t);
 }
 
 __global__ void f64_to_f32_kernel(const double *__restrict__ a, uint64_t n, float *__restrict__ b) {
@@ -795,18 +960,18 @@ __global__ void f64_to_f32_kernel(const double *__restrict__ a, uint64_t n, floa
   if (i < n) b[i] = (float)a[i];
 }
 
-__global__ void targets_kernel(const double *__restrict__ uN, const double *__restrict__ uD, uint64_t n,
-                               double s0, double s1, float *__restrict__ e) {
+// Training targets: psi scaled, for both network outputs.
+__global__ void targets_kernel(const double *__restrict__ v, uint64_t n, double s,
+                               float *__restrict__ e) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  e[2 * i] = (float)(uN[i] / s0);
-  e[2 * i + 1] = (float)(uD[i] / s1);
+  e[2 * i] = (float)(v[i] / s);
+  e[2 * i + 1] = (float)(v[i] / s);
 }
 
 }  // namespace dev
 
 namespace {
-constexpr double SUR_DCAP = 12.0;  // distances are resolved up to this
 
 // Device buffer released on scope exit (error paths included).
 template <class T>
@@ -825,260 +990,305 @@ double secs_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 unsigned nblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+void far_eval(const Engine *E, const Tgt &T, const double *d_xyz, uint64_t n, double *d_val,
+              double *d_grad, hipStream_t st) {
+  FarGrid g{};
+  const uint64_t blocks = std::min<uint64_t>((n + 3) / 4, (uint64_t)E->cus * 16);
+  hipLaunchKernelGGL(far_eval_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, st, T, d_xyz, g, n,
+                     d_val, d_grad);
+  HIP_OK(hipGetLastError());
+}
+
 }  // namespace
+
+bool split_capable(const Engine *E) { return (E->gk == 2 && E->ft == 6) || (E->gk == 1 && E->ft == 4); }
 
 void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrogate_report &rep,
                             float *params_out) {
-  if (E->gk != 2 || E->ft != 6)
-    throw Error(MCCS_ERR_UNSUPPORTED, "the far-field surrogate is built for the N2 probe, force type 6");
+  if (!split_capable(E))
+    throw Error(MCCS_ERR_UNSUPPORTED,
+                "the far-field surrogate is built for N2 force type 6 and He/Ar/co2 force type 4");
   HIP_OK(hipSetDevice(E->device));
   std::memset(&rep, 0, sizeof rep);
   const Tgt &T = E->T;
-  const double shell = o.shell > 0 ? o.shell : 8.0, blend = o.blend > 0 ? o.blend : 2.0;
-  const double h = o.grid > 0 ? o.grid : 0.25;
-  const uint64_t ntr = o.n_train ? o.n_train : 2000000ull, nval = 20000;
-  if (shell + blend + 2 * h > SUR_DCAP)
-    throw Error(MCCS_ERR_INVALID, "shell + blend must stay below 11 A");
+  const double sw1 = o.shell > 0 ? o.shell : 8.0, sw2 = sw1 + (o.blend > 0 ? o.blend : 4.0);
+  const double h = o.grid > 0 ? o.grid : 0.5;
+  if (sw2 > T.rc) throw Error(MCCS_ERR_INVALID, "shell + blend must not exceed the Coulomb cutoff");
+  if (h < 0.05 || h > 2.0) throw Error(MCCS_ERR_INVALID, "grid spacing must lie in [0.05, 2] A");
+  if (o.table_source < 0 || o.table_source > 2) throw Error(MCCS_ERR_INVALID, "table_source must be 0, 1 or 2");
+  const bool network = o.table_source == 0;
+  const uint64_t ntr = network ? (o.n_train ? o.n_train : 2000000ull) : 0, nval = 20000;
   hipStream_t st = E->stream;
   auto t0 = std::chrono::steady_clock::now();
 
-  // 1. points outside (shell - 1.5 A) of every atom and within the longest
-  //    cutoff of some atom (beyond it the field is exactly zero), uniform
-  const double marg = T.rlong + 1.0;
-  double lo[3], hi[3];
-  for (int d = 0; d < 3; d++) {
-    lo[d] = T.blo[d] - marg;
-    hi[d] = T.bhi[d] + marg;
-  }
-  const double dmin = shell - 1.5;
+  // the split target: the pair loop reaches sw2 (and the LJ cutoff)
+  Tgt S = T;
+  S.sw1 = sw1;
+  S.sw2 = sw2;
+  S.isw = 1.0 / (sw2 - sw1);
+  S.sw2cut = sqrt_cut(sw2);
+  S.rlong = std::max(T.rlj, sw2);
+  // the lists of the exact kernel's capacity hold the shorter split lists
+  E->shmem_split = E->shmem;
+
+  // grid: the target's box plus the cutoff and six nodes of margin (the
+  // spline prefilter takes the field as zero beyond the grid; it is, and its
+  // coefficients decay by sqrt(3) - 2 per node from where it ends)
+  FarGrid g{};
+  const double marg = T.rc + 6.0 * h;
+  g.h = h;
+  g.ox = T.blo[0] - marg;
+  g.oy = T.blo[1] - marg;
+  g.oz = T.blo[2] - marg;
+  g.nx = (int)std::ceil((T.bhi[0] - T.blo[0] + 2.0 * marg) / h) + 1;
+  g.ny = (int)std::ceil((T.bhi[1] - T.blo[1] + 2.0 * marg) / h) + 1;
+  g.nz = (int)std::ceil((T.bhi[2] - T.blo[2] + 2.0 * marg) / h) + 1;
+  const uint64_t nodes = (uint64_t)g.nx * g.ny * g.nz;
+  if (E->d_psi) HIP_OK(hipFree(E->d_psi));
+  E->d_psi = nullptr;
+  E->sur_built = false;
+  HIP_OK(hipMalloc(&E->d_psi, sizeof(float) * nodes));
+  DevBuf<double> node64(nodes);  // node values, then coefficients, in FP64
+
+  // sample points where probes go: at least 2 A from every atom and within
+  // the cutoff of some atom (psi vanishes beyond), uniform in the box;
+  // nval of them are held out for validation
+  const double dmin = 2.0;
   const uint64_t want = ntr + nval, chunk = 1u << 20;
   std::vector<double> pts;
   pts.reserve(3 * want);
   std::mt19937_64 rng(o.seed ? o.seed : 2104);
   std::uniform_real_distribution<double> U01(0.0, 1.0);
-  std::vector<float> hp(3 * chunk), hd(chunk);
-  std::vector<double> cand(3 * chunk);
-  DevBuf<double> dx(3 * want), fN(3 * want), uN(want), fD(3 * want), uD(want);
   {
-  DevBuf<float> dp(3 * chunk), dd(chunk);
-  for (int guard = 0; pts.size() < 3 * want; guard++) {
-    if (guard > 1000) throw Error(MCCS_ERR_INVALID, "no room outside the shell to sample");
-    for (uint64_t i = 0; i < chunk; i++)
-      for (int d = 0; d < 3; d++) {
-        cand[3 * i + d] = lo[d] + (hi[d] - lo[d]) * U01(rng);
-        hp[3 * i + d] = (float)cand[3 * i + d];
-      }
-    HIP_OK(hipMemcpyAsync(dp, hp.data(), sizeof(float) * 3 * chunk, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(mindist_kernel, dim3(nblk(chunk)), dim3(256), 0, st, T, (const float *)dp,
-                       (uint64_t)chunk, T.rlong + 1.0, dd);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(hd.data(), dd, sizeof(float) * chunk, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    for (uint64_t i = 0; i < chunk && pts.size() < 3 * want; i++)
-      if (hd[i] >= dmin && hd[i] < T.rlong)  // outside the shell, inside the cutoff
-        for (int d = 0; d < 3; d++) pts.push_back((double)hp[3 * i + d]);  // the float point itself
+    std::vector<float> hp(3 * chunk), hd(chunk);
+    DevBuf<float> dp(3 * chunk), dd(chunk);
+    for (int guard = 0; pts.size() < 3 * want; guard++) {
+      if (guard > 1000) throw Error(MCCS_ERR_INVALID, "no room around the target to sample");
+      for (uint64_t i = 0; i < chunk; i++)
+        for (int d = 0; d < 3; d++)
+          hp[3 * i + d] = (float)(T.blo[d] - T.rc + (T.bhi[d] - T.blo[d] + 2.0 * T.rc) * U01(rng));
+      HIP_OK(hipMemcpyAsync(dp, hp.data(), sizeof(float) * 3 * chunk, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(mindist_kernel, dim3(nblk(chunk)), dim3(256), 0, st, T, (const float *)dp,
+                         (uint64_t)chunk, T.rc + 1.0, dd);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpyAsync(hd.data(), dd, sizeof(float) * chunk, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (uint64_t i = 0; i < chunk && pts.size() < 3 * want; i++)
+        if (hd[i] >= dmin && hd[i] < T.rc)
+          for (int d = 0; d < 3; d++) pts.push_back((double)hp[3 * i + d]);
+    }
   }
-  }
-  // exact energies (and forces, for validation) of both site roles
+  DevBuf<double> dx(3 * want), pv(want), pg(3 * want);
   HIP_OK(hipMemcpyAsync(dx, pts.data(), sizeof(double) * 3 * want, hipMemcpyHostToDevice, st));
-  eval_sites_dev(E, MCCS_SITE_N, dx, want, fN, uN);
-  eval_sites_dev(E, MCCS_SITE_DUMMY, dx, want, fD, uD);
-  std::vector<double> hu[2], hf[2];
-  for (int r = 0; r < 2; r++) {
-    hu[r].resize(want);
-    hf[r].resize(3 * want);
-  }
-  HIP_OK(hipMemcpyAsync(hu[0].data(), uN, sizeof(double) * want, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(hu[1].data(), uD, sizeof(double) * want, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(hf[0].data(), fN, sizeof(double) * 3 * want, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(hf[1].data(), fD, sizeof(double) * 3 * want, hipMemcpyDeviceToHost, st));
+  far_eval(E, S, dx, want, pv, pg, st);
+  std::vector<double> hv(want), hg(3 * want);
+  HIP_OK(hipMemcpyAsync(hv.data(), pv, sizeof(double) * want, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(hg.data(), pg, sizeof(double) * 3 * want, hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
   rep.seconds_sample = secs_since(t0);
 
-  // 2. network: scales from the data, Fourier frequencies N(0, (L / flen)^2)
-  //    in units of the scaled input (1 / flen per A)
-  MlpSpec spec{};
-  double L = 0.0;
-  for (int d = 0; d < 3; d++) {
-    spec.centre[d] = 0.5 * (lo[d] + hi[d]);
-    L = std::max(L, 0.5 * (hi[d] - lo[d]));
-  }
-  spec.in_scale = 1.0 / L;
-  for (int r = 0; r < 2; r++) {
-    double ss = 0.0;
-    for (uint64_t i = 0; i < ntr; i++) ss += hu[r][i] * hu[r][i];
-    spec.out_scale[r] = std::max(std::sqrt(ss / (double)ntr), 1e-6);
-  }
-  std::normal_distribution<double> N01(0.0, 1.0);
-  // feature length: the wavelength scale of the random Fourier features,
-  // by default 0.23 of the domain's extent (the target's largest extent plus
-  // the cutoff on both sides): 18-20 A for a 1,200-atom globule, 40 A for
-  // the 50,000-atom ellipsoid, the values that fit those fields best
-  double ext = 0.0;
-  for (int d = 0; d < 3; d++) ext = std::max(ext, T.bhi[d] - T.blo[d]);
-  const double flen = o.feature_length > 0 ? o.feature_length : 0.23 * (ext + 2.0 * T.rlong);
-  for (int f = 0; f < MLP_NF; f++)
-    for (int d = 0; d < 3; d++) spec.freq[f][d] = (float)(N01(rng) * L / flen);
-  std::vector<float> params(MLP_NPARAM, 0.0f);
-  auto init = [&](size_t off, int fin, int fout) {  // uniform +-sqrt(6 / (fin + fout))
-    const double a = std::sqrt(6.0 / (fin + fout));
-    for (size_t i = 0; i < (size_t)fin * fout; i++) params[off + i] = (float)(a * (2.0 * U01(rng) - 1.0));
-  };
-  init(MLP_W1, MLP_IN, MLP_H);
-  init(MLP_W2, MLP_H, MLP_H);
-  init(MLP_W3, MLP_H, MLP_H);
-  init(MLP_W4, MLP_H, MLP_OUT);
-  if (E->mlp) mlp_destroy(E->mlp);
-  E->mlp = nullptr;
-  E->mlp = mlp_create(E->device, spec, params.data(), 65536);
-
-  // 3. train on the first ntr points (device-resident FP32 positions and
-  //    scaled energies)
-  auto t1 = std::chrono::steady_clock::now();
-  DevBuf<float> txyz(3 * ntr), te(2 * ntr), tdir(3 * ntr), tder(2 * ntr);
-  double wd = 1.0;
-  {  // a random direction per point and the exact directional derivatives
-     // -F . dir of both outputs (scaled), matched by central differences
-    std::vector<float> dir(3 * ntr), der(2 * ntr);
-    double ss = 0.0;
-    for (uint64_t i = 0; i < ntr; i++) {
-      double v[3], nn = 0.0;
-      do {
-        nn = 0.0;
-        for (int d = 0; d < 3; d++) {
-          v[d] = N01(rng);
-          nn += v[d] * v[d];
-        }
-      } while (nn < 1e-12);
-      nn = std::sqrt(nn);
-      for (int d = 0; d < 3; d++) dir[3 * i + d] = (float)(v[d] / nn);
-      for (int r = 0; r < 2; r++) {
-        double dd = 0.0;
-        for (int d = 0; d < 3; d++) dd -= hf[r][3 * i + d] * (double)dir[3 * i + d];
-        der[2 * i + r] = (float)(dd / spec.out_scale[r]);
-        ss += (double)der[2 * i + r] * der[2 * i + r];
-      }
-    }
-    wd = 2.0 * ntr / std::max(ss, 1e-30);  // both terms of comparable size
-    HIP_OK(hipMemcpy(tdir, dir.data(), sizeof(float) * 3 * ntr, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(tder, der.data(), sizeof(float) * 2 * ntr, hipMemcpyHostToDevice));
-  }
-  hipLaunchKernelGGL(f64_to_f32_kernel, dim3(nblk(3 * ntr)), dim3(256), 0, st, (const double *)dx,
-                     (uint64_t)(3 * ntr), txyz);
-  hipLaunchKernelGGL(targets_kernel, dim3(nblk(ntr)), dim3(256), 0, st, (const double *)uN,
-                     (const double *)uD, ntr, spec.out_scale[0], spec.out_scale[1], te);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipStreamSynchronize(st));
-  TrainOpts to;
-  to.epochs = o.epochs > 0 ? o.epochs : 300;
-  to.batch = o.batch > 0 ? o.batch : 8192;
-  to.lr = o.lr > 0 ? o.lr : 5e-3;
-  to.lr_final = to.lr / 20.0;
-  to.seed = o.seed ? o.seed : 2104;
-  to.d_dir = tdir;
-  to.d_deriv = tder;
-  to.fd_h = 0.2;
-  to.w_deriv = wd;
-  rep.train_loss = mlp_train(E->mlp, txyz, te, ntr, to, nullptr);
-  rep.seconds_train = secs_since(t1);
-
-  // 4. held-out error of energies and forces (F = -grad U)
-  {
-    std::vector<double> y(2 * nval), dy(6 * nval);
-    mlp_eval(E->mlp, pts.data() + 3 * ntr, nval, y.data(), dy.data());
-    for (int r = 0; r < 2; r++) {
-      double eu = 0, su = 0, ef = 0, sf = 0;
-      for (uint64_t i = 0; i < nval; i++) {
-        const double ue = hu[r][ntr + i];
-        eu += (y[2 * i + r] - ue) * (y[2 * i + r] - ue);
-        su += ue * ue;
-        for (int d = 0; d < 3; d++) {
-          const double fe = hf[r][3 * (ntr + i) + d], fm = -dy[6 * i + 3 * r + d];
-          ef += (fm - fe) * (fm - fe);
-          sf += fe * fe;
-        }
-      }
-      rep.val_rmse_u[r] = std::sqrt(eu / nval);
-      rep.val_rms_u[r] = std::sqrt(su / nval);
-      rep.val_rmse_f[r] = std::sqrt(ef / (3.0 * nval));
-      rep.val_rms_f[r] = std::sqrt(sf / (3.0 * nval));
-    }
-  }
-
-  // 5. tabulate: energies and capped nearest-atom distance on the grid
   auto t2 = std::chrono::steady_clock::now();
-  const double gm = T.rlong + 3.0 * h;
-  double go[3];
-  int gn[3];
-  for (int d = 0; d < 3; d++) {
-    go[d] = T.blo[d] - gm;
-    gn[d] = (int)std::ceil((T.bhi[d] - T.blo[d] + 2.0 * gm) / h) + 1;
-  }
-  const uint64_t nodes = (uint64_t)gn[0] * gn[1] * gn[2];
-  if (E->d_sg) hipFree(E->d_sg);
-  E->d_sg = nullptr;
-  HIP_OK(hipMalloc(&E->d_sg, sizeof(float4) * nodes));
-  const uint64_t gch = 1u << 20;
-  DevBuf<float> gp(3 * gch), gd(gch), gy(2 * gch);
-  const bool exact_src = o.table_source == 1;
-  DevBuf<double> g64(exact_src ? 3 * gch : 1), gf(exact_src ? 3 * gch : 1),
-      gu0(exact_src ? gch : 1), gu1(exact_src ? gch : 1);
-  // the network is tapered to zero over the last 4 A before the cutoff so
-  // that, splines included, it vanishes where the exact field does
-  const float tap0 = exact_src ? 1e30f : (float)(T.rlong - 5.0);
-  const float tap1 = exact_src ? 2e30f : (float)(T.rlong - 1.0);
-  for (uint64_t f0 = 0; f0 < nodes; f0 += gch) {
-    const uint64_t c = std::min(gch, nodes - f0);
-    hipLaunchKernelGGL(grid_points_kernel, dim3(nblk(c)), dim3(256), 0, st, f0, c, go[0], go[1], go[2], h,
-                       gn[1], gn[2], gp);
-    hipLaunchKernelGGL(mindist_kernel, dim3(nblk(c)), dim3(256), 0, st, T, (const float *)gp, c,
-                       T.rlong + 1.0, gd);
+  MlpSpec spec{};
+  if (network) {
+    // network: psi from its position; Fourier frequencies N(0, (L / flen)^2)
+    // in units of the scaled input (1 / flen per A)
+    auto t1 = std::chrono::steady_clock::now();
+    double L = 0.0;
+    for (int d = 0; d < 3; d++) {
+      spec.centre[d] = 0.5 * (T.blo[d] + T.bhi[d]);
+      L = std::max(L, 0.5 * (T.bhi[d] - T.blo[d]) + T.rc);
+    }
+    spec.in_scale = 1.0 / L;
+    double ss = 0.0;
+    for (uint64_t i = 0; i < ntr; i++) ss += hv[i] * hv[i];
+    spec.out_scale[0] = spec.out_scale[1] = std::max(std::sqrt(ss / (double)ntr), 1e-9);
+    std::normal_distribution<double> N01(0.0, 1.0);
+    double ext = 0.0;
+    for (int d = 0; d < 3; d++) ext = std::max(ext, T.bhi[d] - T.blo[d]);
+    const double flen = o.feature_length > 0 ? o.feature_length : 0.23 * (ext + 2.0 * T.rc);
+    for (int f = 0; f < MLP_NF; f++)
+      for (int d = 0; d < 3; d++) spec.freq[f][d] = (float)(N01(rng) * L / flen);
+    std::vector<float> params(MLP_NPARAM, 0.0f);
+    auto init = [&](size_t off, int fin, int fout) {  // uniform +-sqrt(6 / (fin + fout))
+      const double a = std::sqrt(6.0 / (fin + fout));
+      for (size_t i = 0; i < (size_t)fin * fout; i++) params[off + i] = (float)(a * (2.0 * U01(rng) - 1.0));
+    };
+    init(MLP_W1, MLP_IN, MLP_H);
+    init(MLP_W2, MLP_H, MLP_H);
+    init(MLP_W3, MLP_H, MLP_H);
+    init(MLP_W4, MLP_H, MLP_OUT);
+    if (E->mlp) mlp_destroy(E->mlp);
+    E->mlp = nullptr;
+    E->mlp = mlp_create(E->device, spec, params.data(), 65536);
+    // training data: positions, scaled psi, and the directional derivative of
+    // psi along a random unit vector per point (matched by central
+    // differences of the network)
+    DevBuf<float> txyz(3 * ntr), te(2 * ntr), tdir(3 * ntr), tder(2 * ntr);
+    double wd = 1.0;
+    {
+      std::vector<float> dir(3 * ntr), der(2 * ntr);
+      double sd = 0.0;
+      for (uint64_t i = 0; i < ntr; i++) {
+        double v[3], nn = 0.0;
+        do {
+          nn = 0.0;
+          for (int d = 0; d < 3; d++) {
+            v[d] = N01(rng);
+            nn += v[d] * v[d];
+          }
+        } while (nn < 1e-12);
+        nn = std::sqrt(nn);
+        double dd = 0.0;
+        for (int d = 0; d < 3; d++) {
+          dir[3 * i + d] = (float)(v[d] / nn);
+          dd += hg[3 * i + d] * (double)dir[3 * i + d];
+        }
+        der[2 * i] = der[2 * i + 1] = (float)(dd / spec.out_scale[0]);
+        sd += 2.0 * (double)der[2 * i] * der[2 * i];
+      }
+      wd = 2.0 * ntr / std::max(sd, 1e-30);  // both terms of comparable size
+      HIP_OK(hipMemcpy(tdir, dir.data(), sizeof(float) * 3 * ntr, hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(tder, der.data(), sizeof(float) * 2 * ntr, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(nblk(3 * ntr)), dim3(256), 0, st, (const double *)dx,
+                       (uint64_t)(3 * ntr), txyz);
+    hipLaunchKernelGGL(targets_kernel, dim3(nblk(ntr)), dim3(256), 0, st, (const double *)pv, ntr,
+                       spec.out_scale[0], te);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
-    if (exact_src) {  // the exact evaluator at the nodes (validation of the tabulated path)
-      hipLaunchKernelGGL(f32_to_f64_kernel, dim3(nblk(3 * c)), dim3(256), 0, st, (const float *)gp,
-                         (uint64_t)(3 * c), g64);
-      eval_sites_dev(E, MCCS_SITE_N, g64, c, gf, gu0);
-      eval_sites_dev(E, MCCS_SITE_DUMMY, g64, c, gf, gu1);
-      hipLaunchKernelGGL(pair_to_f32_kernel, dim3(nblk(c)), dim3(256), 0, st, (const double *)gu0,
-                         (const double *)gu1, c, gy);
-    } else {
-      mlp_eval_device(E->mlp, gp, c, gy, nullptr);
+    TrainOpts to;
+    to.epochs = o.epochs > 0 ? o.epochs : 300;
+    to.batch = o.batch > 0 ? o.batch : 8192;
+    to.lr = o.lr > 0 ? o.lr : 5e-3;
+    to.lr_final = to.lr / 20.0;
+    to.seed = o.seed ? o.seed : 2104;
+    to.d_dir = tdir;
+    to.d_deriv = tder;
+    to.fd_h = 0.2;
+    to.w_deriv = wd;
+    rep.train_loss = mlp_train(E->mlp, txyz, te, ntr, to, nullptr);
+    rep.seconds_train = secs_since(t1);
+    // held-out error of the network itself (output 0 and its gradient)
+    std::vector<double> y(2 * nval), dy(6 * nval);
+    mlp_eval(E->mlp, pts.data() + 3 * ntr, nval, y.data(), dy.data());
+    double eu = 0, su = 0, ef = 0, sf = 0;
+    for (uint64_t i = 0; i < nval; i++) {
+      const double ue = hv[ntr + i];
+      eu += (y[2 * i] - ue) * (y[2 * i] - ue);
+      su += ue * ue;
+      for (int d = 0; d < 3; d++) {
+        const double ge = hg[3 * (ntr + i) + d], gm = dy[6 * i + d];
+        ef += (gm - ge) * (gm - ge);
+        sf += ge * ge;
+      }
     }
-    hipLaunchKernelGGL(pack_grid_kernel, dim3(nblk(c)), dim3(256), 0, st, (const float *)gy,
-                       (const float *)gd, c, tap0, tap1, (float)SUR_DCAP, E->d_sg + f0);
+    rep.val_rmse_u[1] = std::sqrt(eu / nval);
+    rep.val_rms_u[1] = std::sqrt(su / nval);
+    rep.val_rmse_f[1] = std::sqrt(ef / (3.0 * nval));
+    rep.val_rms_f[1] = std::sqrt(sf / (3.0 * nval));
+    t2 = std::chrono::steady_clock::now();
+    // tabulate the network at the nodes (MFMA inference), tapered to zero
+    // over the last 3 A before the cutoff distance from every atom
+    const uint64_t gch = 1u << 20;
+    DevBuf<float> gp(3 * gch), gd(gch), gy(2 * gch);
+    for (uint64_t f0 = 0; f0 < nodes; f0 += gch) {
+      const uint64_t c = std::min(gch, nodes - f0);
+      hipLaunchKernelGGL(grid_points_kernel, dim3(nblk(c)), dim3(256), 0, st, f0, c, g, gp);
+      hipLaunchKernelGGL(mindist_kernel, dim3(nblk(c)), dim3(256), 0, st, T, (const float *)gp, c,
+                         T.rc + 1.0, gd);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipStreamSynchronize(st));
+      mlp_eval_device(E->mlp, gp, c, gy, nullptr);
+      hipLaunchKernelGGL(pack_network_kernel, dim3(nblk(c)), dim3(256), 0, st, (const float *)gy,
+                         (const float *)gd, c, (float)(T.rc - 3.0), (float)T.rc, node64 + f0);
+      HIP_OK(hipGetLastError());
+    }
+  } else if (o.table_source == 2) {
+    // no far field: the near terms alone (the ablation the network is
+    // measured against)
+    HIP_OK(hipMemsetAsync((double *)node64, 0, sizeof(double) * nodes, st));
+  } else {
+    // the exact field at every node (the linked-cell evaluator)
+    const uint64_t blocks = std::min<uint64_t>((nodes + 3) / 4, (uint64_t)E->cus * 16);
+    hipLaunchKernelGGL(far_eval_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, st, S,
+                       (const double *)nullptr, g, nodes, (double *)node64, (double *)nullptr);
     HIP_OK(hipGetLastError());
   }
+  // node values -> interpolating B-spline coefficients, axis by axis
+  {
+    const uint64_t nxy = (uint64_t)g.nx * g.ny, nyz = (uint64_t)g.ny * g.nz;
+    hipLaunchKernelGGL(bspline_prefilter_kernel, dim3(nblk(nxy)), dim3(256), 0, st, (double *)node64, g.nz,
+                       nxy, (uint64_t)1, (uint64_t)1);
+    hipLaunchKernelGGL(bspline_prefilter_kernel, dim3(nblk((uint64_t)g.nx * g.nz)), dim3(256), 0, st,
+                       (double *)node64, g.ny, (uint64_t)g.nx * g.nz, (uint64_t)g.nz, (uint64_t)g.nz);
+    hipLaunchKernelGGL(bspline_prefilter_kernel, dim3(nblk(nyz)), dim3(256), 0, st, (double *)node64, g.nx,
+                       nyz, nyz, nyz);
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(nblk(nodes)), dim3(256), 0, st, (const double *)node64,
+                       nodes, E->d_psi);
+    HIP_OK(hipGetLastError());
+  }
+  S.psi = E->d_psi;
+  S.pox = g.ox;
+  S.poy = g.oy;
+  S.poz = g.oz;
+  S.pih = 1.0 / h;
+  S.pnx = g.nx;
+  S.pny = g.ny;
+  S.pnz = g.nz;
   HIP_OK(hipStreamSynchronize(st));
   rep.seconds_table = secs_since(t2);
+  // held-out error of the table the trajectories use (against the exact field)
+  {
+    DevBuf<double> sv(nval), sg(3 * nval);
+    hipLaunchKernelGGL(spline_eval_kernel, dim3(nblk(nval)), dim3(256), 0, st, S,
+                       (const double *)(dx + 3 * ntr), nval, sv, sg);
+    HIP_OK(hipGetLastError());
+    std::vector<double> v(nval), gg(3 * nval);
+    HIP_OK(hipMemcpyAsync(v.data(), sv, sizeof(double) * nval, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(gg.data(), sg, sizeof(double) * 3 * nval, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    double eu = 0, su = 0, ef = 0, sf = 0;
+    for (uint64_t i = 0; i < nval; i++) {
+      const double ue = hv[ntr + i];
+      eu += (v[i] - ue) * (v[i] - ue);
+      su += ue * ue;
+      for (int d = 0; d < 3; d++) {
+        const double ge = hg[3 * (ntr + i) + d];
+        ef += (gg[3 * i + d] - ge) * (gg[3 * i + d] - ge);
+        sf += ge * ge;
+      }
+    }
+    rep.val_rmse_u[0] = std::sqrt(eu / nval);
+    rep.val_rms_u[0] = std::sqrt(su / nval);
+    rep.val_rmse_f[0] = std::sqrt(ef / (3.0 * nval));
+    rep.val_rms_f[0] = std::sqrt(sf / (3.0 * nval));
+  }
   rep.n_train = ntr;
   rep.grid_nodes = nodes;
-  for (int f = 0; f < MLP_NF; f++)
-    for (int d = 0; d < 3; d++) rep.spec.freq[f][d] = spec.freq[f][d];
-  for (int d = 0; d < 3; d++) rep.spec.centre[d] = spec.centre[d];
-  rep.spec.in_scale = spec.in_scale;
-  rep.spec.out_scale[0] = spec.out_scale[0];
-  rep.spec.out_scale[1] = spec.out_scale[1];
-  if (params_out) mlp_get_params(E->mlp, params_out);
-
-  E->T.sox = go[0];
-  E->T.soy = go[1];
-  E->T.soz = go[2];
-  E->T.sih = 1.0 / h;
-  E->T.snx = gn[0];
-  E->T.sny = gn[1];
-  E->T.snz = gn[2];
-  E->T.sd0 = shell;
-  E->T.sd1 = shell + blend;
-  E->T.sdcap = SUR_DCAP;
+  if (network) {
+    for (int f = 0; f < MLP_NF; f++)
+      for (int d = 0; d < 3; d++) rep.spec.freq[f][d] = spec.freq[f][d];
+    for (int d = 0; d < 3; d++) rep.spec.centre[d] = spec.centre[d];
+    rep.spec.in_scale = spec.in_scale;
+    rep.spec.out_scale[0] = spec.out_scale[0];
+    rep.spec.out_scale[1] = spec.out_scale[1];
+    if (params_out) mlp_get_params(E->mlp, params_out);
+  }
+  E->Ts = S;
   E->sur_built = true;
+  if (E->split) {  // rebuilt while in use: the kernel sees the new table
+    HIP_OK(hipMemcpy(E->d_tg, &E->Ts, sizeof(Tgt), hipMemcpyHostToDevice));
+  }
 }
 
 void engine_use_surrogate(Engine *E, bool on) {
   HIP_OK(hipSetDevice(E->device));
   if (on && !E->sur_built) throw Error(MCCS_ERR_INVALID, "no surrogate has been built for this engine");
-  E->T.sg = on ? E->d_sg : nullptr;
-  HIP_OK(hipMemcpy(E->d_tg, &E->T, sizeof(Tgt), hipMemcpyHostToDevice));
+  if (on && E->lanes != 64)
+    throw Error(MCCS_ERR_INVALID, "the far-field surrogate runs 64 lanes per trajectory");
+  E->split = on;
+  HIP_OK(hipMemcpy(E->d_tg, on ? &E->Ts : &E->T, sizeof(Tgt), hipMemcpyHostToDevice));
 }
 
 // ---------------------------------------------------------------- batches
@@ -1091,6 +1301,7 @@ namespace {
 
 struct BatchClass {
   int gk = 1, ft = 2;
+  bool split = false;        // members run with their range-split far fields
   std::vector<int> members;  // target indices, in queue order
   int lanes = 64, block = 256;
   double skin = 1.0;
@@ -1119,6 +1330,7 @@ std::map<std::string, std::pair<int, double>> &tune_cache() {
 struct Batch {
   int device = 0, cus = 0;
   bool count = true, use_graph = true;
+  bool split = false;  // far-field surrogates in use (where built)
   int lanes_opt = 0;
   double skin_opt = 0.0;
   hipStream_t stream = nullptr;
@@ -1128,8 +1340,11 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint64_t> gkey;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // orders batch_run_device after / before a caller's stream
   ~Batch() {
     hipSetDevice(device);
+    if (ev_in) hipEventDestroy(ev_in);
+    if (ev_out) hipEventDestroy(ev_out);
     if (gexec) hipGraphExecDestroy(gexec);
     if (graph) hipGraphDestroy(graph);
     for (auto &c : cls) {
@@ -1145,7 +1360,7 @@ struct Batch {
 
 static std::string class_signature(const Batch &B, const BatchClass &c) {
   std::ostringstream o;
-  o << c.gk << "/" << c.ft;
+  o << c.gk << "/" << c.ft << (B.split ? "/far" : "");
   for (int t : c.members) {
     const Tgt &T = B.eng[t]->T;
     int lg = 0;
@@ -1158,30 +1373,36 @@ static std::string class_signature(const Batch &B, const BatchClass &c) {
 // (Re)builds a class's device target array for its launch shape: list
 // capacity for the shape's LDS budget and the chosen skin.
 static void configure_class(Batch &B, BatchClass &c) {
-  c.fn = c.gk == 1 ? select_traj_multi1(c.ft, c.lanes, B.count)
-                   : (c.gk == 2 ? select_traj_multi2(c.ft, c.lanes, B.count) : nullptr);
+  c.split = B.split;
+  for (int t : c.members) c.split = c.split && B.eng[t]->sur_built;
+  if (c.split) c.lanes = 64;  // the far-field kernels run whole wavefronts
+  c.fn = c.gk == 1 ? select_traj_multi1(c.ft, c.lanes, B.count, c.split)
+                   : (c.gk == 2 ? select_traj_multi2(c.ft, c.lanes, B.count, c.split) : nullptr);
   if (!c.fn)
     throw Error(MCCS_ERR_UNSUPPORTED,
                 "batched runs support He/Ar/co2 force types 2 and 4 and N2 force type 6");
   const int groups = c.block / c.lanes;
   const bool central = c.gk == 3;
+  const size_t entry = sizeof(int);
   const size_t base = lds_bytes(0, 0, c.block, c.lanes, false, central);
   // half a CU per block when every member's estimated lists, scaled from a
-  // 1 A skin to the class's, fit in 90 % of it; else the whole CU
-  const size_t half = base < LDS_BLOCK_HALF ? (LDS_BLOCK_HALF - base) / (sizeof(int) * groups) : 0;
+  // 1 A skin to the class's (and to the split radius), fit in 90 % of it;
+  // else the whole CU
+  const size_t half = base < LDS_BLOCK_HALF ? (LDS_BLOCK_HALF - base) / (entry * groups) : 0;
   bool fit_half = true;
   for (int t : c.members) {
     const Engine *e = B.eng[t];
-    const double r1 = e->T.rlong + 1.0 + LIST_OFF_MAX, rs = e->T.rlong + c.skin + LIST_OFF_MAX;
+    const double r1 = e->T.rlong + 1.0 + LIST_OFF_MAX;
+    const double rs = (c.split ? e->Ts.rlong : e->T.rlong) + c.skin + LIST_OFF_MAX;
     fit_half = fit_half && e->list_need * (rs / r1) * (rs / r1) * (rs / r1) <= 0.9 * (double)half;
   }
   const size_t budget = fit_half ? LDS_BLOCK_HALF : LDS_BLOCK_FULL;
   const size_t room = base < budget ? budget - base : 0;
-  const int cap = (int)(room / (sizeof(int) * groups));
+  const int cap = (int)(room / (entry * groups));
   std::vector<Tgt> tg;
   int maxcap = 0;
   for (int t : c.members) {
-    Tgt T = B.eng[t]->T;
+    Tgt T = c.split ? B.eng[t]->Ts : B.eng[t]->T;
     T.list_cap = std::min(T.natoms, cap);
     T.skin = c.skin;
     maxcap = std::max(maxcap, T.list_cap);
@@ -1279,7 +1500,7 @@ static bool stage_samples(Batch &B, const uint64_t *n, const double *const *samp
       if (n[t] && !samples[t]) throw Error(MCCS_ERR_INVALID, "null samples");
       if (n[t])
         HIP_OK(hipMemcpyAsync(c.d_u + 9 * off[m], samples[t], sizeof(double) * 9 * n[t],
-                              hipMemcpyHostToDevice, B.stream));
+                              hipMemcpyDefault, B.stream));  // host or device samples
     }
   }
   return changed;
@@ -1313,6 +1534,7 @@ static void run_classes(Batch &B, bool sizes_changed) {
     key.push_back(c.n);
     key.push_back((uint64_t)c.lanes);
     key.push_back((uint64_t)(c.skin * 1000.0));
+    key.push_back((uint64_t)c.split);
   }
   if (!B.gexec || sizes_changed || key != B.gkey) {
     if (B.gexec) HIP_OK(hipGraphExecDestroy(B.gexec));
@@ -1359,6 +1581,37 @@ void batch_run(Batch *B, const uint64_t *n, const double *const *samples,
   fetch_records(*B, n, records);
 }
 
+// The same over device-resident samples and records, enqueued after the
+// work already on `stream` and finished before later work on it (the batch
+// runs on its own stream, which waits on and then releases the caller's).
+void batch_run_device(Batch *B, const uint64_t *n, const double *const *samples_dev,
+                      mccs_traj_record *const *records_dev, void *stream) {
+  HIP_OK(hipSetDevice(B->device));
+  hipStream_t s = stream ? (hipStream_t)stream : B->stream;
+  if (!B->ev_in) {
+    HIP_OK(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
+  }
+  if (s != B->stream) {
+    HIP_OK(hipEventRecord(B->ev_in, s));
+    HIP_OK(hipStreamWaitEvent(B->stream, B->ev_in, 0));
+  }
+  const bool changed = stage_samples(*B, n, samples_dev);
+  run_classes(*B, changed);
+  for (auto &c : B->cls)
+    for (size_t m = 0; m < c.members.size(); m++) {
+      const int t = c.members[m];
+      if (n[t] && !records_dev[t]) throw Error(MCCS_ERR_INVALID, "null records");
+      if (n[t])
+        HIP_OK(hipMemcpyAsync(records_dev[t], c.d_rec + c.toff[m], sizeof(mccs_traj_record) * n[t],
+                              hipMemcpyDeviceToDevice, B->stream));
+    }
+  if (s != B->stream) {
+    HIP_OK(hipEventRecord(B->ev_out, B->stream));
+    HIP_OK(hipStreamWaitEvent(s, B->ev_out, 0));
+  }
+}
+
 void batch_last_work(Batch *B, uint64_t out[5]) {
   HIP_OK(hipSetDevice(B->device));
   HIP_OK(hipStreamSynchronize(B->stream));
@@ -1384,8 +1637,14 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
   bool first = true;
   for (auto &c : B->cls) {
     std::vector<std::pair<int, double>> cand;
+    const bool split = B->split && [&] {
+      for (int t : c.members)
+        if (!B->eng[t]->sur_built) return false;
+      return true;
+    }();
     for (int L : lanes_c)
       for (double sk : skin_c) {
+        if (split && L != 64) continue;  // the far-field kernels run whole wavefronts
         if (B->lanes_opt && L != B->lanes_opt) continue;
         if (B->skin_opt > 0.0 && sk != B->skin_opt) continue;
         cand.push_back({L, sk});
@@ -1447,6 +1706,38 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
   B->graph = nullptr;
   B->gkey.clear();
   return rep.str();
+}
+
+// Far-field surrogates of every member with a dipole field (N2 type 6,
+// He/Ar/co2 type 4), one build per target (the others run exact); reports
+// (may be null) receive one report per target, zero for the others.
+void batch_build_surrogate(Batch *B, const mccs_surrogate_opts &o, mccs_surrogate_report *reports) {
+  for (size_t t = 0; t < B->eng.size(); t++) {
+    mccs_surrogate_report r{};
+    if (split_capable(B->eng[t])) engine_build_surrogate(B->eng[t], o, r, nullptr);
+    if (reports) reports[t] = r;
+  }
+  if (B->split) {
+    for (auto &c : B->cls) configure_class(*B, c);
+    B->gkey.clear();
+  }
+}
+
+void batch_use_surrogate(Batch *B, bool on) {
+  HIP_OK(hipSetDevice(B->device));
+  B->split = on;
+  for (auto &c : B->cls) {
+    // tuned shapes are cached per signature, which names the far field
+    auto it = tune_cache().find(class_signature(*B, c));
+    if (it != tune_cache().end() && !B->lanes_opt && !(B->skin_opt > 0.0)) {
+      c.lanes = it->second.first;
+      c.skin = it->second.second;
+    } else if (!B->lanes_opt) {
+      c.lanes = 64;
+    }
+    configure_class(*B, c);
+  }
+  B->gkey.clear();
 }
 
 void batch_shapes(const Batch *B, int32_t *lanes, double *skin) {

@@ -32,7 +32,8 @@ void engine_tally_device(Engine *e, const mccs_traj_record *rec_dev, uint64_t n_
                          uint64_t n_traj, double *omega_dev, int64_t *counts_dev,
                          double *mean_err_dev, void *stream);
 
-// Far-field surrogate of an N2 type-6 engine (trains and tabulates on the GPU).
+// Range-split far field of an N2 type-6 or He/Ar/co2 type-4 engine (the
+// exact field or the network trained on it, tabulated on the GPU).
 void engine_build_surrogate(Engine *e, const mccs_surrogate_opts &o, mccs_surrogate_report &rep,
                             float *params_out);
 void engine_use_surrogate(Engine *e, bool on);
@@ -44,8 +45,12 @@ Batch *batch_create(const Prepared *const *targets, int ntargets, const mccs_bat
 void batch_destroy(Batch *b);
 void batch_run(Batch *b, const uint64_t *n, const double *const *samples,
                mccs_traj_record *const *records);
+void batch_run_device(Batch *b, const uint64_t *n, const double *const *samples_dev,
+                      mccs_traj_record *const *records_dev, void *stream);
 std::string batch_autotune(Batch *b, const uint64_t *n_trial, const double *const *samples);
 void batch_last_work(Batch *b, uint64_t out[5]);
 void batch_shapes(const Batch *b, int32_t *lanes, double *skin);
+void batch_build_surrogate(Batch *b, const mccs_surrogate_opts &o, mccs_surrogate_report *reports);
+void batch_use_surrogate(Batch *b, bool on);
 
 }  // namespace mccs

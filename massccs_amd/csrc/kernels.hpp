@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -87,14 +88,23 @@ struct Tgt {
   double rlong;             // longest cutoff of the run's terms
   double skin;              // Verlet-list skin (A)
   int co2fix;               // CO2: chi from the true initial velocity (not src/System.cpp:1012)
-  // far-field surrogate (N2, force type 6): cubic B-spline coefficients of
-  // the network's two site energies and of the distance to the nearest atom
-  // (x = E_N, y = E_dummy, z = d) on a grid; null when not in use
-  const float4 *sg;
-  double sox, soy, soz, sih;  // grid origin and 1 / spacing
-  int snx, sny, snz;
-  double sd0, sd1;          // exact below sd0, blended to the surrogate at sd1
-  double sdcap;             // distance stored for nodes at or beyond the cap
+  // Range-split far field (null when not in use).  Every electrostatic pair
+  // term (shifted Coulomb and the smoothed dipole field) is split by a
+  // partition of unity S(r) of the pair distance: S = 1 below sw1, 0 beyond
+  // sw2, a quintic smoothstep between.  The pair loop sums S times the
+  // reference's term over the atoms within sw2 (every pair nearer than sw1
+  // exact); the remainder, (1 - S) times the same term summed over every
+  // atom, is one smooth scalar field psi(x) = sum_j q_j (1 - S) u_c(r_j) with
+  // u_c = 1/r - 1.5/rc + r^2/(2 rc^3), whose value, gradient and Hessian give
+  // the Coulomb energy and force of a charged site (q K psi, -q K grad psi)
+  // and the far part of the dipole field (E = -grad psi, grad E = -Hess psi).
+  // psi is read from cubic B-spline coefficients on a regular grid (FP32:
+  // their rounding, 6e-8 of psi, stays far below the interpolation error).
+  const float *psi;
+  double pox, poy, poz, pih;  // grid origin and 1 / spacing
+  int pnx, pny, pnz;
+  double sw1, sw2, isw;       // switch start and end, 1 / (sw2 - sw1)
+  double sw2cut;              // pairs with r2 < sw2cut carry a near term
 };
 
 // ---------------------------------------------------------------- probes
@@ -303,7 +313,7 @@ __device__ __forceinline__ uint32_t count_of(bool c) {
   return UNI ? (uint32_t)__popcll(__ballot(c)) : (uint32_t)c;
 }
 
-template <class S, bool COUNT, bool UNI = false>
+template <class S, bool COUNT, bool UNI = false, bool SPLIT = false>
 __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS][3],
                                            const double (&qs)[S::NS], const double4 at,
                                            const double2 lp, const double2 lc, int allow_lj,
@@ -327,8 +337,8 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
       if (S::coul(s) || S::dip(s)) in_c = r2 < T.rc2cut;
       if (COUNT) {
         if (S::lj(s)) w.lj += count_of<UNI>(in_lj);
-        if (S::coul(s)) w.coul += count_of<UNI>(in_c);
-        if (S::dip(s)) w.dip += count_of<UNI>(in_c);
+        if (!SPLIT && S::coul(s)) w.coul += count_of<UNI>(in_c);
+        if (!SPLIT && S::dip(s)) w.dip += count_of<UNI>(in_c);
       }
       // Coulomb and dipole terms without a branch: most listed atoms are
       // inside their cutoff, and outside it the weights (qq, A, B) are zero,
@@ -349,7 +359,52 @@ __device__ __forceinline__ void pair_terms(const Tgt &T, const double (&p)[S::NS
           a[s].fy += flj * dy;
           a[s].fz += flj * dz;
         }
-        if (any_c) {
+        if (any_c && SPLIT) {
+          // near part of the range-split terms: the reference's shifted
+          // Coulomb potential q u(r), u = 1/r - 1.5/rc + r^2/(2 rc^3), and its
+          // smoothed dipole field (the gradient of u) times the partition S(r);
+          // for S = 1 (r <= sw1) these are the reference's terms exactly
+          const bool in_s = r2 < T.sw2cut;
+          if (COUNT) {
+            if (S::coul(s)) w.coul += count_of<UNI>(in_s);
+            if (S::dip(s)) w.dip += count_of<UNI>(in_s);
+          }
+          const double r = r2 * rinv, r3 = rinv * r2inv;
+          const double t = fmin(fmax((r - T.sw1) * T.isw, 0.0), 1.0), t2 = t * t, omt = 1.0 - t;
+          const double Sv = 1.0 - t2 * t * fma(t, fma(6.0, t, -15.0), 10.0);
+          const double S1 = -30.0 * t2 * omt * omt * T.isw;
+          const double S2 = -60.0 * t * omt * (1.0 - 2.0 * t) * (T.isw * T.isw);
+          const double uc = rinv + T.cc1 + T.cc2 * r2;  // u
+          const double u1 = r * T.irc3 - r2inv;         // u'
+          const double u2 = 2.0 * r3 + T.irc3;          // u''
+          const double ph1 = S1 * uc + Sv * u1;
+          const double ph2 = S2 * uc + 2.0 * S1 * u1 + Sv * u2;
+          // per unit charge: potential, E = Ar d, grad E = Ar I + Br d d^T
+          const double ph = in_s ? Sv * uc : 0.0;
+          const double Ar = in_s ? -ph1 * rinv : 0.0;
+          const double Br = in_s ? (ph1 * rinv - ph2) * r2inv : 0.0;
+          if (S::coul(s)) {
+            const double qq = qs[s] * qk;
+            a[s].U += qq * ph;
+            const double fc = qq * Ar;
+            a[s].fx += fc * dx;
+            a[s].fy += fc * dy;
+            a[s].fz += fc * dz;
+          }
+          if (S::dip(s)) {
+            const double A = at.w * Ar, B = at.w * Br;
+            a[s].Ex += dx * A;
+            a[s].Ey += dy * A;
+            a[s].Ez += dz * A;
+            const double bx = B * dx, by = B * dy, bz = B * dz;
+            a[s].Exx += A + bx * dx;
+            a[s].Eyy += A + by * dy;
+            a[s].Ezz += A + bz * dz;
+            a[s].Exy += bx * dy;
+            a[s].Exz += bx * dz;
+            a[s].Eyz += by * dz;
+          }
+        } else if (any_c) {
           // shifted force r^-3 (1 - r^3/rc^3) = r^-3 - rc^-3 (src/Force.cpp:671-683 and
           // the smoothed field of :393-426)
           const double r3 = rinv * r2inv;
@@ -620,8 +675,9 @@ __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, con
   // within R from the back (entry t >= nin is at cap - 1 - (t - nin))
   auto take = [&](bool valid, int a) {
     bool kin = false, kout = false;
+    double4 at = make_double4(0.0, 0.0, 0.0, 0.0);
     if (valid) {
-      const double4 at = LDS ? sh.atoms[a] : T.atoms[a];
+      at = LDS ? sh.atoms[a] : T.atoms[a];
       const double ex = at.x - c[0], ey = at.y - c[1], ez = at.z - c[2];
       const double d2 = ex * ex + ey * ey + ez * ez;
       kin = d2 < Rin2;
@@ -686,11 +742,138 @@ __device__ void build_list(const Tgt &T, const Shm &sh, int lane, VList &vl, con
   vl.valid = nin + nout <= cap;
 }
 
+// Uniform cubic B-spline basis piece i (0..3) at t in [0, 1), with its first
+// and second derivatives in t.  Branch-free (the lanes of a wavefront take
+// different pieces): every piece is formed and the lane's is selected.
+__device__ __forceinline__ void bspline_piece(int i, double t, double &w, double &dw, double &ddw) {
+  const double u = 1.0 - t, t2 = t * t, t3 = t2 * t;
+  const double w0 = u * u * u * (1.0 / 6.0), w1 = (3.0 * t3 - 6.0 * t2 + 4.0) * (1.0 / 6.0);
+  const double w2 = (-3.0 * t3 + 3.0 * t2 + 3.0 * t + 1.0) * (1.0 / 6.0), w3 = t3 * (1.0 / 6.0);
+  const double d0 = -0.5 * u * u, d1 = 1.5 * t2 - 2.0 * t, d2 = -1.5 * t2 + t + 0.5, d3 = 0.5 * t2;
+  const double e0 = u, e1 = 3.0 * t - 2.0, e2 = 1.0 - 3.0 * t, e3 = t;
+  w = i == 0 ? w0 : (i == 1 ? w1 : (i == 2 ? w2 : w3));
+  dw = i == 0 ? d0 : (i == 1 ? d1 : (i == 2 ? d2 : d3));
+  ddw = i == 0 ? e0 : (i == 1 ? e1 : (i == 2 ? e2 : e3));
+}
+
+// The same in FP32 (the far field's interpolation, whose own error is
+// ~1e-3 of the field, runs in single precision).
+__device__ __forceinline__ void bspline_piece_f(int i, float t, float &w, float &dw, float &ddw) {
+  const float u = 1.0f - t, t2 = t * t, t3 = t2 * t;
+  const float w0 = u * u * u * (1.0f / 6.0f), w1 = (3.0f * t3 - 6.0f * t2 + 4.0f) * (1.0f / 6.0f);
+  const float w2 = (-3.0f * t3 + 3.0f * t2 + 3.0f * t + 1.0f) * (1.0f / 6.0f), w3 = t3 * (1.0f / 6.0f);
+  const float d0 = -0.5f * u * u, d1 = 1.5f * t2 - 2.0f * t, d2 = -1.5f * t2 + t + 0.5f, d3 = 0.5f * t2;
+  const float e0 = u, e1 = 3.0f * t - 2.0f, e2 = 1.0f - 3.0f * t, e3 = t;
+  w = i == 0 ? w0 : (i == 1 ? w1 : (i == 2 ? w2 : w3));
+  dw = i == 0 ? d0 : (i == 1 ? d1 : (i == 2 ? d2 : d3));
+  ddw = i == 0 ? e0 : (i == 1 ? e1 : (i == 2 ? e2 : e3));
+}
+
+// Far part of the range-split field at the sites: every lane of a 64-lane
+// group takes one of the 4 x 4 x 4 B-spline nodes around each site.  The
+// coefficients are loaded first (far_load, issued before the pair loop so
+// that the loads overlap it) and applied after it (far_apply): each lane adds
+// its node's share of psi and its derivatives to the site's accumulators,
+// which the group reduction then sums together with the near terms.  A
+// charged site gets q K psi and -q K grad psi, a polarisable one
+// E -= grad psi and grad E -= Hess psi.  Nodes outside the grid hold zero
+// (the field vanishes there).
+struct FarNode {
+  float t[3];   // position within the cell of the site, in grid units
+  float c;      // this lane's coefficient
+  bool on;
+};
+
+// The coefficients a trajectory read at its last evaluation, by site: a probe
+// moves about a tenth of a grid spacing per step, so most steps need the same
+// 4 x 4 x 4 nodes again.  b[s][0] = INT_MIN marks an empty entry.
+template <int NS>
+struct FarCache {
+  int b[NS][3];
+  float c[NS];
+  __device__ void clear() {
+    for (int s = 0; s < NS; s++) b[s][0] = INT_MIN;
+  }
+};
+
+template <class S>
+__device__ __forceinline__ void far_load(const Tgt &T, int lane, const double (&p)[S::NS][3],
+                                         FarNode (&fn)[S::NS], int only,
+                                         FarCache<S::NS> *cache = nullptr) {
+#pragma unroll
+  for (int s = 0; s < S::NS; s++) {
+    fn[s].on = false;
+    fn[s].c = 0.0f;
+    if (!(S::coul(s) || S::dip(s))) continue;
+    if (only >= 0 && s != only) continue;
+    const double ux = (p[s][0] - T.pox) * T.pih, uy = (p[s][1] - T.poy) * T.pih,
+                 uz = (p[s][2] - T.poz) * T.pih;
+    if (!(fabs(ux) < 1e9 && fabs(uy) < 1e9 && fabs(uz) < 1e9)) continue;  // parked or non-finite
+    const double fx = floor(ux), fy = floor(uy), fz = floor(uz);
+    fn[s].on = true;
+    fn[s].t[0] = (float)(ux - fx);
+    fn[s].t[1] = (float)(uy - fy);
+    fn[s].t[2] = (float)(uz - fz);
+    const int bx = (int)fx - 1, by = (int)fy - 1, bz = (int)fz - 1;
+    if (cache && cache->b[s][0] == bx && cache->b[s][1] == by && cache->b[s][2] == bz) {
+      fn[s].c = cache->c[s];
+      continue;
+    }
+    const long ix = (long)bx + (lane >> 4), iy = (long)by + ((lane >> 2) & 3), iz = (long)bz + (lane & 3);
+    if (ix >= 0 && ix < T.pnx && iy >= 0 && iy < T.pny && iz >= 0 && iz < T.pnz)
+      fn[s].c = T.psi[((size_t)ix * T.pny + (size_t)iy) * T.pnz + (size_t)iz];
+    if (cache) {
+      cache->b[s][0] = bx;
+      cache->b[s][1] = by;
+      cache->b[s][2] = bz;
+      cache->c[s] = fn[s].c;
+    }
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void far_apply(const Tgt &T, int lane, const FarNode (&fn)[S::NS],
+                                          const double (&qs)[S::NS], Acc (&a)[S::NS]) {
+  const float h1 = (float)T.pih, h2 = h1 * h1;
+#pragma unroll
+  for (int s = 0; s < S::NS; s++) {
+    if (!(S::coul(s) || S::dip(s))) continue;
+    if (!fn[s].on) continue;
+    const float c = fn[s].c;
+    float wx, dwx, ddwx, wy, dwy, ddwy, wz, dwz, ddwz;
+    bspline_piece_f(lane >> 4, fn[s].t[0], wx, dwx, ddwx);
+    bspline_piece_f((lane >> 2) & 3, fn[s].t[1], wy, dwy, ddwy);
+    bspline_piece_f(lane & 3, fn[s].t[2], wz, dwz, ddwz);
+    const float cyz = c * wy * wz, cxz = c * wx * wz, cxy = c * wx * wy;
+    const double gx = (double)(dwx * cyz * h1), gy = (double)(dwy * cxz * h1),
+                 gz = (double)(dwz * cxy * h1);
+    if (S::coul(s)) {
+      const double qk = qs[s] * KC;
+      a[s].U += qk * (double)(wx * cyz);
+      a[s].fx -= qk * gx;
+      a[s].fy -= qk * gy;
+      a[s].fz -= qk * gz;
+    }
+    if (S::dip(s)) {
+      a[s].Ex -= gx;
+      a[s].Ey -= gy;
+      a[s].Ez -= gz;
+      a[s].Exx -= (double)(ddwx * cyz * h2);
+      a[s].Eyy -= (double)(ddwy * cxz * h2);
+      a[s].Ezz -= (double)(ddwz * cxy * h2);
+      a[s].Exy -= (double)(dwx * dwy * (c * wz) * h2);
+      a[s].Exz -= (double)(dwx * dwz * (c * wy) * h2);
+      a[s].Eyz -= (double)(dwy * dwz * (c * wx) * h2);
+    }
+  }
+}
+
 // Forces and energies of all sites at positions p (group-uniform result).
-template <class S, int L, bool LDS, bool COUNT>
+template <class S, int L, bool LDS, bool COUNT, bool SPLIT = false>
 __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p)[S::NS][3],
                          const double (&qs)[S::NS], double (&f)[S::NS][3], double (&U)[S::NS],
-                         Work &w, VList *vl) {
+                         Work &w, VList *vl, FarCache<S::NS> *fcache = nullptr) {
+  static_assert(!SPLIT || L == 64, "the far field is evaluated by whole wavefronts");
   constexpr bool CEN = any_central<S>();
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
@@ -704,7 +887,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     double4 at;
     double2 lp, lc;
     fetch<LDS, CEN>(T, sh, ai, at, lp, lc);
-    pair_terms<S, COUNT, L == 64>(T, p, qs, at, lp, lc, first, a, w);
+    pair_terms<S, COUNT, L == 64, SPLIT>(T, p, qs, at, lp, lc, first, a, w);
   };
   RunGen g;
   if (!S::CUT) {
@@ -768,13 +951,18 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
         // terms are computed (hides the LDS/L2 latency of the gather)
         double4 at = make_double4(0.0, 0.0, 0.0, 0.0);
         double2 lp = make_double2(0.0, 0.0), lc = make_double2(0.0, 0.0);
+        // the far field's coefficients are loaded before the pair loop (their
+        // latency overlaps it) and applied after it
+        FarNode fn[S::NS];
+        if constexpr (SPLIT) far_load<S>(T, lane, p, fn, -1, fcache);
         if (lane < n) fetch<LDS, CEN>(T, sh, entry(lane), at, lp, lc);
         for (int t = lane; t < n; t += L) {
           const double4 cat = at;
           const double2 clp = lp, clc = lc;
           if (t + L < n) fetch<LDS, CEN>(T, sh, entry(t + L), at, lp, lc);
-          pair_terms<S, COUNT, L == 64>(T, p, qs, cat, clp, clc, t < nin, a, w);
+          pair_terms<S, COUNT, L == 64, SPLIT>(T, p, qs, cat, clp, clc, t < nin, a, w);
         }
+        if constexpr (SPLIT) far_apply<S>(T, lane, fn, qs, a);
         reduce<S, L>(a, sh.red);
         finish<S>(T, a, -1, f, U);
         return;
@@ -795,6 +983,11 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     g.iy0 = iy0;
     g.nys = iy1 - iy0 + 1;
     walk_runs<L>(T, sh, lane, g, (ix1 - ix0 + 1) * g.nys, visit);
+    if constexpr (SPLIT) {
+      FarNode fn[S::NS];
+      far_load<S>(T, lane, p, fn, -1);
+      far_apply<S>(T, lane, fn, qs, a);
+    }
     reduce<S, L>(a, sh.red);
     finish<S>(T, a, -1, f, U);
     return;
@@ -827,9 +1020,14 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
       double4 at;
       double2 lp, lc;
       fetch<LDS, CEN>(T, sh, ai, at, lp, lc);
-      pair_terms<S, COUNT, L == 64>(T, pp, qs, at, lp, lc, first, b, w);
+      pair_terms<S, COUNT, L == 64, SPLIT>(T, pp, qs, at, lp, lc, first, b, w);
     };
     walk_runs<L>(T, sh, lane, g, nr, visit1);
+    if constexpr (SPLIT) {
+      FarNode fn[S::NS];
+      far_load<S>(T, lane, pp, fn, ps);
+      far_apply<S>(T, lane, fn, qs, b);
+    }
     reduce<S, L>(b, sh.red);
     finish<S>(T, b, ps, f, U);
   }
@@ -1092,7 +1290,6 @@ struct GState {
   unsigned long long j;         // trajectory (index over all targets of the launch)
   int t;                        // its target
   int k;                        // try (restart) index of the trajectory, 0..TRIES_MAX-1
-  double sur[3][8];             // surrogate per site: U, grad U, d, grad d
   unsigned long long slot;      // spawn slot taken while its item is not yet written
   int steps;                    // steps of this try
   int cont;                     // 1: the group goes on with try k of trajectory j
@@ -1186,6 +1383,7 @@ __host__ __device__ inline size_t lds_bytes(int natoms, int list_cap, int block,
     b += (sizeof(double4) + sizeof(double2) * (central ? 2 : 1)) * (size_t)natoms;
   b += group_scratch(lanes) * (size_t)(block / lanes);
   b += sizeof(GState) * (size_t)(block / lanes);
+  b = (b + 31) / 32 * 32;
   b += lidx * (size_t)list_cap * (size_t)(block / lanes);
   return b + 64;
 }
@@ -1203,7 +1401,8 @@ __device__ void stage(const Tgt &T, Shm &sh, char *smem) {
   p += group_scratch(L) * (blockDim.x / L);
   GState *states = reinterpret_cast<GState *>(p);
   p += sizeof(GState) * (blockDim.x / L);
-  char *lists = p;
+  // the lists start on a 32-byte boundary (double4 entries of the copied lists)
+  char *lists = smem + ((size_t)(p - smem) + 31) / 32 * 32;
   if (LDS) {
     for (int i = threadIdx.x; i < T.natoms; i += blockDim.x) {
       atoms[i] = T.atoms[i];
@@ -1221,64 +1420,6 @@ __device__ void stage(const Tgt &T, Shm &sh, char *smem) {
   sh.red = reinterpret_cast<double *>(sh.incl);
   sh.state = states + g;
   sh.list = lists + sizeof(LIdx<LDS>) * (size_t)T.list_cap * g;
-}
-
-// ---------------------------------------------------------------- surrogate
-// Uniform cubic B-spline weights of the 4 nodes around t in [0, 1) and their
-// derivatives.
-__device__ __forceinline__ void bsw(double t, double w[4], double dw[4]) {
-  const double u = 1.0 - t, t2 = t * t, t3 = t2 * t;
-  w[0] = u * u * u * (1.0 / 6.0);
-  w[1] = (3.0 * t3 - 6.0 * t2 + 4.0) * (1.0 / 6.0);
-  w[2] = (-3.0 * t3 + 3.0 * t2 + 3.0 * t + 1.0) * (1.0 / 6.0);
-  w[3] = t3 * (1.0 / 6.0);
-  dw[0] = -0.5 * u * u;
-  dw[1] = 1.5 * t2 - 2.0 * t;
-  dw[2] = -1.5 * t2 + t + 0.5;
-  dw[3] = 0.5 * t2;
-}
-
-// Surrogate energy of one site (role 0: N, 1: dummy) and the distance field,
-// with gradients, at p; the 64 spline nodes are spread over the group.
-template <int L>
-__device__ void spline_site(const Tgt &T, int lane, const double p[3], int role, double &U,
-                            double gU[3], double &d, double gd[3]) {
-  const double ux = (p[0] - T.sox) * T.sih, uy = (p[1] - T.soy) * T.sih, uz = (p[2] - T.soz) * T.sih;
-  const double fx = floor(ux), fy = floor(uy), fz = floor(uz);
-  double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
-  bsw(ux - fx, wx, dx);
-  bsw(uy - fy, wy, dy);
-  bsw(uz - fz, wz, dz);
-  const long ix0 = (long)fx - 1, iy0 = (long)fy - 1, iz0 = (long)fz - 1;
-  double aU = 0, ax = 0, ay = 0, az = 0, ad = 0, bx = 0, by = 0, bz = 0;
-  for (int nd = lane; nd < 64; nd += L) {
-    const int a = nd >> 4, b = (nd >> 2) & 3, c = nd & 3;
-    const long ix = ix0 + a, iy = iy0 + b, iz = iz0 + c;
-    double v = 0.0, dv = T.sdcap;
-    if (ix >= 0 && ix < T.snx && iy >= 0 && iy < T.sny && iz >= 0 && iz < T.snz) {
-      const float4 g = T.sg[((size_t)ix * T.sny + iy) * T.snz + iz];
-      v = role ? (double)g.y : (double)g.x;
-      dv = (double)g.z;
-    }
-    const double w = wx[a] * wy[b] * wz[c];
-    const double gx = dx[a] * wy[b] * wz[c], gy = wx[a] * dy[b] * wz[c], gz = wx[a] * wy[b] * dz[c];
-    aU += w * v;
-    ax += gx * v;
-    ay += gy * v;
-    az += gz * v;
-    ad += w * dv;
-    bx += gx * dv;
-    by += gy * dv;
-    bz += gz * dv;
-  }
-  U = gsum<L>(aU);
-  gU[0] = gsum<L>(ax) * T.sih;
-  gU[1] = gsum<L>(ay) * T.sih;
-  gU[2] = gsum<L>(az) * T.sih;
-  d = gsum<L>(ad);
-  gd[0] = gsum<L>(bx) * T.sih;
-  gd[1] = gsum<L>(by) * T.sih;
-  gd[2] = gsum<L>(bz) * T.sih;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -1347,14 +1488,14 @@ __device__ __attribute__((noinline)) bool resolve(const Queue &Q, mccs_traj_reco
 // (trajectory, try) until every trajectory is resolved; per loop iteration
 // every working group performs exactly one force evaluation: the initial
 // force of a try, or the force of one velocity-Verlet/RATTLE step.
-template <int GK, int FT, int L, bool LDS, bool COUNT, bool MT>
+template <int GK, int FT, int L, bool LDS, bool COUNT, bool MT, bool SPLIT = false>
 __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu(traj_waves(LDS)))) void traj_kernel(
     const Tgt *__restrict__ tg, const unsigned long long *__restrict__ toff, int ntg,
     const double *__restrict__ u, uint64_t n, mccs_traj_record *__restrict__ rec, Queue Q) {
   static_assert(!(MT && LDS), "targets staged in LDS need a launch of their own");
   using S = Spec<GK, FT>;
   constexpr bool CEN = any_central<S>();
-  extern __shared__ __align__(16) char smem[];
+  extern __shared__ __align__(32) char smem[];
   Shm sh;
   stage<L, LDS, CEN>(tg[0], sh, smem);
   const int lane = threadIdx.x % L;
@@ -1385,6 +1526,8 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
   Work w{0, 0, 0, 0};
   int status = ST_NEED;
   unsigned long long wait_since = 0;  // wall clock when the group began waiting, 0 if not
+  FarCache<S::NS> fcache;  // far-field coefficients of the last evaluation (range split)
+  fcache.clear();
   if (lane == 0) {
     st.vl.valid = -1;
     st.cont = 0;
@@ -1497,6 +1640,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
       }
       wait_since = 0;
       w = Work{0, 0, 0, 0};
+      fcache.clear();  // a new try, possibly of another target
       if (lane == 0) {
         st.vl.valid = -1;  // each try builds its own neighbour list: results do not
                            // depend on what the group ran before
@@ -1581,51 +1725,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
       for (int s = 0; s < S::NS; s++)
         for (int d = 0; d < 3; d++) p[s][d] = st.pos[s][d];
-      bool exact = true;
-      if constexpr (GK == 2 && FT == 6) {
-        if (T.sg) {
-          // far-field surrogate: exact evaluation only when a site is closer
-          // to the target than the end of the blend band (values kept in LDS
-          // across the exact evaluation)
-          exact = false;
-#pragma unroll
-          for (int s = 0; s < S::NS; s++) {
-            double u, gu[3], dd, gd[3];
-            spline_site<L>(T, lane, p[s], s == 2 ? 1 : 0, u, gu, dd, gd);
-            exact = exact || dd < T.sd1;
-            if (lane == 0) {
-              st.sur[s][0] = u;
-              st.sur[s][4] = dd;
-              for (int d = 0; d < 3; d++) {
-                st.sur[s][1 + d] = gu[d];
-                st.sur[s][5 + d] = gd[d];
-              }
-            }
-          }
-          lds_fence();
-        }
-      }
-      if (exact) evaluate<S, L, LDS, COUNT>(T, sh, lane, p, qs, f, U, w, &st.vl);
-      if constexpr (GK == 2 && FT == 6) {
-        if (T.sg) {
-#pragma unroll
-          for (int s = 0; s < S::NS; s++) {
-            const double su = st.sur[s][0], sdd = st.sur[s][4];
-            if (sdd >= T.sd1) {  // surrogate only
-              U[s] = su;
-              for (int d = 0; d < 3; d++) f[s][d] = -st.sur[s][1 + d];
-            } else if (sdd > T.sd0) {  // blend: U = S Uex + (1 - S) Usur
-              const double wdt = T.sd1 - T.sd0, t = (sdd - T.sd0) / wdt;
-              const double Sw = 1.0 - t * t * (3.0 - 2.0 * t);
-              const double dS = -6.0 * t * (1.0 - t) / wdt;
-              const double dU = U[s] - su;
-              for (int d = 0; d < 3; d++)
-                f[s][d] = Sw * f[s][d] - (1.0 - Sw) * st.sur[s][1 + d] - dU * dS * st.sur[s][5 + d];
-              U[s] = Sw * U[s] + (1.0 - Sw) * su;
-            }
-          }
-        }
-      }
+      evaluate<S, L, LDS, COUNT, SPLIT>(T, sh, lane, p, qs, f, U, w, &st.vl, &fcache);
 #pragma unroll
       for (int s = 0; s < S::NS; s++) Ut += U[s];
       if (lane == 0) {
@@ -1886,6 +1986,22 @@ TrajFn pick_traj_full(int L, bool lds, bool count) {
   }
 }
 
+// Range-split far field (force types with a dipole field: He/Ar/co2 type 4,
+// N2 type 6): 64 lanes.
+template <int GK, int FT>
+TrajFn pick_traj_split(int L, bool lds, bool count) {
+  if (L != 64) return nullptr;
+  if (lds)
+    return count ? traj_kernel<GK, FT, 64, true, true, false, true> : traj_kernel<GK, FT, 64, true, false, false, true>;
+  return count ? traj_kernel<GK, FT, 64, false, true, false, true> : traj_kernel<GK, FT, 64, false, false, false, true>;
+}
+
+template <int GK, int FT>
+TrajFn pick_traj_multi_split(int L, bool count) {
+  if (L != 64) return nullptr;
+  return count ? traj_kernel<GK, FT, 64, false, true, true, true> : traj_kernel<GK, FT, 64, false, false, true, true>;
+}
+
 template <int GK, int FT>
 TrajFn pick_traj_basic(int L, bool lds, bool count) {
   if (L != 64 || !count) return nullptr;
@@ -1902,11 +2018,11 @@ TrajFn pick_traj_multi(int L, bool count) {
   }
 }
 
-TrajFn select_traj_gk1(int ft, int L, bool lds, bool count);
-TrajFn select_traj_gk2(int ft, int L, bool lds, bool count);
+TrajFn select_traj_gk1(int ft, int L, bool lds, bool count, bool split);
+TrajFn select_traj_gk2(int ft, int L, bool lds, bool count, bool split);
 TrajFn select_traj_gk3(int ft, int L, bool lds, bool count);
-TrajFn select_traj_multi1(int ft, int L, bool count);
-TrajFn select_traj_multi2(int ft, int L, bool count);
+TrajFn select_traj_multi1(int ft, int L, bool count, bool split);
+TrajFn select_traj_multi2(int ft, int L, bool count, bool split);
 
 }  // namespace dev
 }  // namespace mccs

@@ -4,7 +4,8 @@
 namespace mccs {
 namespace dev {
 
-TrajFn select_traj_gk1(int ft, int L, bool lds, bool count) {
+TrajFn select_traj_gk1(int ft, int L, bool lds, bool count, bool split) {
+  if (split) return ft == 4 ? pick_traj_split<1, 4>(L, lds, count) : nullptr;
   switch (ft) {
     case 2: return pick_traj_full<1, 2>(L, lds, count);
     case 4: return pick_traj_full<1, 4>(L, lds, count);

@@ -4,7 +4,8 @@
 namespace mccs {
 namespace dev {
 
-TrajFn select_traj_gk2(int ft, int L, bool lds, bool count) {
+TrajFn select_traj_gk2(int ft, int L, bool lds, bool count, bool split) {
+  if (split) return ft == 6 ? pick_traj_split<2, 6>(L, lds, count) : nullptr;
   switch (ft) {
     case 6: return pick_traj_full<2, 6>(L, lds, count);
     case 1: return pick_traj_basic<2, 1>(L, lds, count);

@@ -315,6 +315,77 @@ static void site_lc(const oracle_ctx *c, const double rp[3], double qi,
     for (int ci = st[index]; ci < st[index + 1]; ci++) {
       int cell = lst[ci];
       int b = c->cell_first[cell], e = b + c->cell_count[cell];
+#ifdef ORACLE_SIMD
+      /* Timing build only (liboracle_fast_*.so, the CPU baseline): the
+       * per-cell atom loops vectorised as the reference's '#pragma omp simd'
+       * loops are under its release flags (src/Force.cpp, e.g. :638-677 for
+       * the first shell with LJ and :680-700 for the second shell without),
+       * with the same terms; the checker build below keeps the sequential
+       * order. */
+      const double rlj = p->lj_cutoff, rc = p->coul_cutoff;
+      const double rc6inv = 1.0 / pow(rlj, 6), rc3inv = 1.0 / pow(rc, 3);
+      double sU = 0, sfx = 0, sfy = 0, sfz = 0, sEx = 0, sEy = 0, sEz = 0;
+      double sExx = 0, sEyy = 0, sEzz = 0, sExy = 0, sExz = 0, sEyz = 0;
+#define ORACLE_ELECTROSTATIC_TERMS                                                         \
+  if (r < rc) {                                                                            \
+    double qj = p->q[t];                                                                   \
+    if (use_coul) {                                                                        \
+      double Ucoul = qi * qj / r * KCOUL;                                                  \
+      double fcoul = Ucoul * r2inv * (1.0 - pow(r / rc, 3));                               \
+      sU += Ucoul - 1.5 * qi * qj * KCOUL / rc + 0.5 * qi * qj * KCOUL * r2 * rc3inv;      \
+      sfx += fcoul * dx;                                                                   \
+      sfy += fcoul * dy;                                                                   \
+      sfz += fcoul * dz;                                                                   \
+    }                                                                                      \
+    if (use_dip) {                                                                         \
+      double r3inv = 1.0 / r * r2inv, r5inv = r3inv * r2inv;                               \
+      double sm = 1.0 - r * r2 * rc3inv;                                                   \
+      double q3 = qj * r3inv * sm, q5 = -3.0 * qj * r5inv * sm, qr = -3.0 * qj * rc3inv * r2inv; \
+      sEx += dx * q3;                                                                      \
+      sEy += dy * q3;                                                                      \
+      sEz += dz * q3;                                                                      \
+      sExx += q3 + dx * dx * (q5 + qr);                                                    \
+      sEyy += q3 + dy * dy * (q5 + qr);                                                    \
+      sEzz += q3 + dz * dz * (q5 + qr);                                                    \
+      sExy += dx * dy * (q5 + qr);                                                         \
+      sExz += dx * dz * (q5 + qr);                                                         \
+      sEyz += dy * dz * (q5 + qr);                                                         \
+    }                                                                                      \
+  }
+      if (pass == 0 && use_lj) {
+#pragma omp simd reduction(+ : sU, sfx, sfy, sfz, sEx, sEy, sEz, sExx, sEyy, sEzz, sExy, sExz, sEyz)
+        for (int t = b; t < e; t++) {
+          double dx = rp[0] - p->x[t], dy = rp[1] - p->y[t], dz = rp[2] - p->z[t];
+          double r2 = dx * dx + dy * dy + dz * dz;
+          double r = sqrt(r2);
+          double r2inv = 1.0 / r2;
+          if (r < rlj) {
+            double r6inv = r2inv * r2inv * r2inv;
+            double lj1 = 4.0 * E[t] * pow(S[t], 6.0);
+            double lj2 = lj1 * pow(S[t], 6.0);
+            double flj = r6inv * (12.0 * lj2 * r6inv - 6.0 * lj1) * r2inv;
+            sU += r6inv * (lj2 * r6inv - lj1) - rc6inv * (lj2 * rc6inv - lj1);
+            sfx += flj * dx;
+            sfy += flj * dy;
+            sfz += flj * dz;
+          }
+          ORACLE_ELECTROSTATIC_TERMS
+        }
+      } else {
+#pragma omp simd reduction(+ : sU, sfx, sfy, sfz, sEx, sEy, sEz, sExx, sEyy, sEzz, sExy, sExz, sEyz)
+        for (int t = b; t < e; t++) {
+          double dx = rp[0] - p->x[t], dy = rp[1] - p->y[t], dz = rp[2] - p->z[t];
+          double r2 = dx * dx + dy * dy + dz * dz;
+          double r = sqrt(r2);
+          double r2inv = 1.0 / r2;
+          ORACLE_ELECTROSTATIC_TERMS
+        }
+      }
+#undef ORACLE_ELECTROSTATIC_TERMS
+      a.U += sU; a.fx += sfx; a.fy += sfy; a.fz += sfz;
+      a.Ex += sEx; a.Ey += sEy; a.Ez += sEz;
+      a.Exx += sExx; a.Eyy += sEyy; a.Ezz += sEzz; a.Exy += sExy; a.Exz += sExz; a.Eyz += sEyz;
+#else
       for (int t = b; t < e; t++) {
         double dx = rp[0] - p->x[t];
         double dy = rp[1] - p->y[t];
@@ -328,6 +399,7 @@ static void site_lc(const oracle_ctx *c, const double rp[3], double qi,
           if (use_dip) dip_cut_term(c, p->q[t], r, r2, dx, dy, dz, &a);
         }
       }
+#endif
     }
   }
   if (use_dip) {
