@@ -232,6 +232,32 @@ mccs_status mccs_tally_device(mccs_engine *e, const mccs_traj_record *records_de
                               int64_t *counts_dev, double *mean_err_dev, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Target preparation steps on the GPU, for very large targets.  mccs_prepare
+ * (host) stays the authoritative path; these compute the same results from
+ * device arrays. */
+
+/* Linked-cell binning and the stable reorder of the atoms by cell
+ * (LinkedCell::calculateAtomsInsideOfCell / sortingAtoms,
+ * src/LinkedCell.cpp:93-225): for n atoms at xyz_dev (n x 3 device doubles,
+ * the oriented target) and the grid of mccs_prepared_info (cell edge, corner,
+ * nx, ny, nz), writes cell_start_dev (nx ny nz + 1 device ints: the first
+ * sorted position of every cell, then n) and order_dev (n device ints: the
+ * input index of the atom at each sorted position; atoms of one cell keep
+ * their input order).  An atom outside the grid returns MCCS_ERR_INVALID, as
+ * the host preparation does.  Synchronises the stream. */
+mccs_status mccs_cells_device(int32_t device, int64_t n, const double *xyz_dev, double cell,
+                              const double corner[3], int32_t nx, int32_t ny, int32_t nz,
+                              int32_t *cell_start_dev, int32_t *order_dev, void *stream);
+
+/* Centre of mass, total mass and inertia tensor about the centre of mass of
+ * n atoms (MoleculeTarget::calculateCenterOfMass and setInertia,
+ * src/MoleculeTarget.cpp:701-745), from device positions and masses;
+ * inertia receives 9 doubles, row-major.  The FP64 sums are reduced in
+ * another order than the host's.  Synchronises the stream. */
+mccs_status mccs_inertia_device(int32_t device, int64_t n, const double *xyz_dev, const double *m_dev,
+                                double com[3], double *mass, double inertia[9], void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Batches: many prepared targets in one run (no counterpart in the
  * reference, which runs one target per process).  Targets are grouped by
  * probe kind and force type; each group runs as one persistent launch whose

@@ -106,6 +106,7 @@ EXPORTS = [
     "mccs_batch_last_work", "mccs_mlp_eval", "mccs_mlp_loss_grad", "mccs_mlp_train",
     "mccs_engine_build_surrogate", "mccs_engine_use_surrogate", "mccs_run_device_noalloc",
     "mccs_batch_build_surrogate", "mccs_batch_use_surrogate", "mccs_batch_run_device",
+    "mccs_cells_device", "mccs_inertia_device",
 ]
 
 _lib = None
@@ -170,6 +171,12 @@ def lib():
     L.mccs_batch_build_surrogate.argtypes = [C.c_void_p, C.POINTER(SurrogateOpts),
                                              C.POINTER(SurrogateReport)]
     L.mccs_batch_use_surrogate.argtypes = [C.c_void_p, C.c_int32]
+    L.mccs_cells_device.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_double,
+                                    C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]
+    L.mccs_inertia_device.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.c_void_p]
     L.mccs_batch_run_device.argtypes = [C.c_void_p, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]
     L.mccs_tally.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
@@ -501,6 +508,26 @@ def mlp_train(spec, params, xyz, e, epochs=10, batch=4096, lr=2e-3, seed=1, devi
     _check(lib().mccs_mlp_train(device, C.byref(spec), _fp(params), _dp(xyz), _dp(e), len(xyz),
                                 epochs, batch, lr, seed, _dp(hist)))
     return params, hist
+
+
+def cells_device(n, xyz_ptr, cell, corner, nx, ny, nz, start_ptr, order_ptr, device=0,
+                 stream=None):
+    """mccs_cells_device over device buffers (pointers): the linked-cell
+    binning and stable reorder of n atoms on the GPU."""
+    c = (C.c_double * 3)(*corner)
+    _check(lib().mccs_cells_device(device, n, C.c_void_p(xyz_ptr), cell, c, nx, ny, nz,
+                                   C.c_void_p(start_ptr), C.c_void_p(order_ptr),
+                                   C.c_void_p(stream) if stream else None))
+
+
+def inertia_device(n, xyz_ptr, m_ptr, device=0, stream=None):
+    """mccs_inertia_device: (centre of mass, total mass, 3 x 3 inertia tensor)."""
+    com = (C.c_double * 3)()
+    mass = C.c_double()
+    I = (C.c_double * 9)()
+    _check(lib().mccs_inertia_device(device, n, C.c_void_p(xyz_ptr), C.c_void_p(m_ptr), com,
+                                     C.byref(mass), I, C.c_void_p(stream) if stream else None))
+    return np.array(com), mass.value, np.array(I).reshape(3, 3)
 
 
 def tally(rec, n_iter, n_traj):

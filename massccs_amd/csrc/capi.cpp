@@ -380,6 +380,28 @@ mccs_status mccs_batch_use_surrogate(mccs_batch *b, int32_t on) {
   });
 }
 
+mccs_status mccs_cells_device(int32_t device, int64_t n, const double *xyz_dev, double cell,
+                              const double corner[3], int32_t nx, int32_t ny, int32_t nz,
+                              int32_t *cell_start_dev, int32_t *order_dev, void *stream) {
+  return guard([&] {
+    need(corner && cell_start_dev && (n == 0 || (xyz_dev && order_dev)), "null argument");
+    need(cell > 0.0 && nx > 0 && ny > 0 && nz > 0, "invalid grid");
+    mccs::device_cell_sort(device, n, xyz_dev, cell, corner, nx, ny, nz, cell_start_dev, order_dev,
+                           stream);
+  });
+}
+
+mccs_status mccs_inertia_device(int32_t device, int64_t n, const double *xyz_dev, const double *m_dev,
+                                double com[3], double *mass, double inertia[9], void *stream) {
+  return guard([&] {
+    need(com && mass && inertia && (n == 0 || (xyz_dev && m_dev)), "null argument");
+    double I[3][3];
+    mccs::device_inertia(device, n, xyz_dev, m_dev, com, mass, I, stream);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) inertia[3 * a + b] = I[a][b];
+  });
+}
+
 // src/System.cpp:287-317
 mccs_status mccs_tally(const mccs_traj_record *rec, uint64_t n_iter, uint64_t n_traj,
                        double *omega_iter, int64_t *counts, double *ccs_mean, double *ccs_err) {

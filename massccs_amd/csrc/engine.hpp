@@ -53,4 +53,11 @@ void batch_shapes(const Batch *b, int32_t *lanes, double *skin);
 void batch_build_surrogate(Batch *b, const mccs_surrogate_opts &o, mccs_surrogate_report *reports);
 void batch_use_surrogate(Batch *b, bool on);
 
+// Target preparation steps on the GPU (prep_device.hip): linked-cell binning
+// with the stable reorder, and centre of mass / inertia tensor.
+void device_cell_sort(int device, int64_t n, const double *xyz_dev, double cell, const double corner[3],
+                      int nx, int ny, int nz, int *cell_start_dev, int *order_dev, void *stream);
+void device_inertia(int device, int64_t n, const double *xyz_dev, const double *m_dev, double com[3],
+                    double *mass, double inertia[3][3], void *stream);
+
 }  // namespace mccs

@@ -806,10 +806,13 @@ static inline double ellipsoid_r(const oracle_params *p, const double r[3]) {
 
 /* The reference loops forever for a probe that never leaves (zero speed or a
  * non-finite state); here such a trajectory is reported lost with tries = -1
- * once one try has run this many steps, as the engine does. */
+ * once one try has run this many steps, as the engine does.  A positive
+ * max_total_steps also ends a trajectory whose steps over all its tries reach
+ * it (the same report). */
 #define ORACLE_MAX_STEPS 4000000
 #define CAPPED(sc, r, v)                                                       \
   ((sc) >= ORACLE_MAX_STEPS ||                                                  \
+   (c->p.max_total_steps > 0 && o->steps >= c->p.max_total_steps) ||           \
    !isfinite((r)[0] + (r)[1] + (r)[2] + (v)[0] + (v)[1] + (v)[2]))
 
 /* System::run_He, src/System.cpp:338-511 (also Ar and 1-site co2) */

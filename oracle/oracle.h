@@ -51,6 +51,10 @@ typedef struct {
   double half_box[3];          /* 0.5 * linked-cell box */
   int co2_vcm_fix;             /* CO2: 1 measures chi from the true initial velocity
                                   (0 = the reference, src/System.cpp:1012) */
+  int max_total_steps;         /* 0: no limit; else a trajectory whose steps over all
+                                  its tries reach this is reported lost with tries = -1
+                                  (a budget for classification runs, not a reference
+                                  behaviour) */
 } oracle_params;
 
 typedef struct oracle_ctx oracle_ctx;

@@ -2,6 +2,7 @@
 prepared target, and the parity criteria."""
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -78,7 +79,7 @@ def prepared(cfg):
     return M.Prepared(M.config_from_dict(cfg))
 
 
-def oracle_for(P, cfg):
+def oracle_for(P, cfg, max_total_steps=0):
     """Oracle context over the product's prepared target (cell-sorted arrays,
     grid, ellipsoid); the preparation itself is pinned separately."""
     i = P.info
@@ -86,7 +87,7 @@ def oracle_for(P, cfg):
     prep = dict(natoms=i.natoms, force_type=i.force_type, grid=[i.nx, i.ny, i.nz, i.ncells],
                 alpha=i.alpha, mu=i.mu, abc=[i.a, i.b, i.c], bmax=i.bmax,
                 halfbox=list(i.half_box), **a)
-    return O.Oracle(prep, cfg)
+    return O.Oracle(prep, cfg, max_total_steps=max_total_steps)
 
 
 def pair_scales(P, cfg, r, qi, lj, coul, dip, cut=True):
@@ -153,15 +154,25 @@ def oracle_refs(P, cfg, u, key=None):
     """Oracle records of samples u at the run's dt, at dt/2, and at dt with
     the impact parameter moved by 4 ulp (cached by key within the test
     session: several tests compare launch variants against the same
-    trajectories)."""
+    trajectories).  The two classification runs stop a trajectory once its
+    steps exceed 2.5 (dt/2) or 1.5 (4 ulp) times the longest trajectory of
+    the dt run, plus 10^4: it is then reported lost (tries = -1), differs
+    from the dt run, and counts as chaotic, which it is, having changed its
+    length that much; the budget keeps an orbit that a chaotic trajectory
+    enters at dt/2 from running for minutes."""
     if key is not None and key in _REF_CACHE:
         return _REF_CACHE[key]
-    orc = oracle_for(P, cfg)
-    ref = orc.run(u)
-    half = oracle_for(P, dict(cfg, dt=float(cfg.get("dt", 10.0)) / 2)).run(u)
+    t0 = time.perf_counter()
+    ref = oracle_for(P, cfg).run(u)
+    print("oracle: %d trajectories at dt in %.1f s" % (len(u), time.perf_counter() - t0), flush=True)
+    longest = int(ref["steps"].max()) if len(ref) else 0
+    half = oracle_for(P, dict(cfg, dt=float(cfg.get("dt", 10.0)) / 2),
+                      max_total_steps=int(2.5 * longest) + 10000).run(u)
+    print("oracle: at dt/2, %.1f s" % (time.perf_counter() - t0), flush=True)
     up = np.array(u, dtype=np.float64, copy=True)
     up[:, 0] *= 1.0 + 4.0 * 2.0 ** -52
-    pert = orc.run(up)
+    pert = oracle_for(P, cfg, max_total_steps=int(1.5 * longest) + 10000).run(up)
+    print("oracle: from b + 4 ulp, %.1f s" % (time.perf_counter() - t0), flush=True)
     if key is not None:
         _REF_CACHE[key] = (ref, half, pert)
     return ref, half, pert

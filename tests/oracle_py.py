@@ -50,6 +50,7 @@ class OracleParams(C.Structure):
         ("temperature", C.c_double), ("dt", C.c_double),
         ("a", C.c_double), ("b", C.c_double), ("c", C.c_double),
         ("bmax", C.c_double), ("half_box", C.c_double * 3), ("co2_vcm_fix", C.c_int),
+        ("max_total_steps", C.c_int),
     ]
 
 
@@ -129,7 +130,7 @@ def input_defaults(cfg):
 class Oracle:
     """Owns an oracle context built from a prepared target."""
 
-    def __init__(self, prep, cfg, variant="strict"):
+    def __init__(self, prep, cfg, variant="strict", max_total_steps=0):
         d = input_defaults(cfg)
         self.L = lib(variant)
         self.prep, self.cfg, self.d = prep, cfg, d
@@ -153,6 +154,7 @@ class Oracle:
         for i in range(3):
             p.half_box[i] = prep["halfbox"][i]
         p.co2_vcm_fix = 1 if cfg.get("CO2 vcm fix", "no") == "yes" else 0
+        p.max_total_steps = int(max_total_steps)
         self.params = p
         self.ctx = self.L.oracle_create(C.byref(p))
 
