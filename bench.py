@@ -194,7 +194,7 @@ def run_headline(args, target):
     flops_p = flops_of(rp)
 
     # surrogate+batched leg: the target as a batch of one, range-split far field
-    B = M.Batch([P], count_work=True, use_graph=True, skin=1.0)
+    B = M.Batch([P], lanes=args.sur_lanes, count_work=True, use_graph=True, skin=1.0)
     t0 = time.perf_counter()
     rep = B.build_surrogate(shell=SUR_SHELL, blend=SUR_BLEND, grid=SUR_GRID, exact_table=True)[0]
     build_s = time.perf_counter() - t0
@@ -245,6 +245,7 @@ def run_headline(args, target):
                          "plain": int(((rp["hit"] == 1) & (rp["success"] == 0)).sum())},
                 "within_0.5%": bool(abs(err) <= 0.005)},
         "surrogate": {"shell_A": SUR_SHELL, "switch_end_A": SUR_SHELL + SUR_BLEND,
+                      "lanes_per_trajectory": args.sur_lanes,
                       "grid_A": SUR_GRID, "table": "exact far field at the nodes",
                       "grid_nodes": rep["grid_nodes"], "build_s": round(build_s, 2),
                       "table_rel_err_psi": rep["val_rmse_u"][0] / max(rep["val_rms_u"][0], 1e-300),
@@ -407,6 +408,55 @@ def run_batch(args):
     }
 
 
+# BF16 MFMA products per network row (64 features -> 3 x 128 SiLU): forward
+# 64x128 + 2 x 128x128; backward data products 2 x 128x128 (the layer-1
+# input gradient is not needed in training); weight gradients 64x128 + 2 x
+# 128x128; 2 flops per multiply-add.
+MLP_FWD_FLOPS = 2 * (64 * 128 + 2 * 128 * 128)
+MLP_TRAIN_FLOPS_PER_ROW = MLP_FWD_FLOPS + 2 * 2 * 128 * 128 + MLP_FWD_FLOPS
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def run_network(args):
+    """The learned far field of the 50k ellipsoid: 2 x 10^6 points sampled by
+    the linked-cell evaluator, trained on the GPU (value and central-difference
+    directional derivative: three network rows per point, batch 8192) and
+    tabulated at every grid node; reports the BF16 MFMA rate of training."""
+    import massccs_amd as M
+    cfg, (s, xyz, q), label = ellipsoid50k()
+    P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
+    E = M.Engine(P, count_work=True)
+    epochs = args.epochs
+    n_train = 2_000_000
+    t0 = time.perf_counter()
+    rep = E.build_surrogate(n_train=n_train, epochs=epochs, shell=SUR_SHELL, blend=SUR_BLEND,
+                            grid=SUR_GRID)[0]
+    build_s = time.perf_counter() - t0
+    rows = 3 * n_train * epochs
+    train_flops = rows * MLP_TRAIN_FLOPS_PER_ROW
+    table_flops = rep["grid_nodes"] * MLP_FWD_FLOPS
+    t_train = rep["seconds_train"]
+    rate = train_flops / t_train / 1e12
+    return {
+        "metric": "far-field network training on 1 MI355X (BF16 MFMA, FP32 accumulation)",
+        "value": round(rows / t_train, 1), "unit": "network rows/s (forward + backward + Adam)",
+        "n_gpus": 1, "steps": epochs, "warmup": 0, "ms_per_step": round(1e3 * t_train / epochs, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (exact far field of the seeded target at sampled points)",
+        "config": {"workload": "network far field, %s + N2: %d points x %d epochs, batch 8192, "
+                               "3 rows per point" % (label, n_train, epochs),
+                   "n_atoms": P.info.natoms, "grid_nodes": rep["grid_nodes"], "parallelism": "1 GPU"},
+        "seconds": {"sample": round(rep["seconds_sample"], 3), "train": round(t_train, 3),
+                    "table": round(rep["seconds_table"], 3), "build": round(build_s, 3)},
+        "roofline": {"bound": "mfma", "achieved": round(rate, 2), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(rate / BF16_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": "whole training (every kernel of the step in the time)",
+                     "flops_train": int(train_flops), "flops_table": int(table_flops)},
+        "network_rel_err": {"psi": rep["val_rmse_u"][1] / max(rep["val_rms_u"][1], 1e-300),
+                            "grad": rep["val_rmse_f"][1] / max(rep["val_rms_f"][1], 1e-300)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,7 +467,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=8.0,
                     help="seconds of CPU work per baseline run (5 runs, median)")
     ap.add_argument("--workload", default="ellipsoid50k",
-                    choices=["ellipsoid50k", "shell1m", "globule", "batch64", "c60", "ubq_he"],
+                    choices=["ellipsoid50k", "shell1m", "globule", "batch64", "c60", "ubq_he",
+                             "network"],
                     help="ellipsoid50k: the headline configuration (plain vs surrogate+batched); "
                          "the others are secondary measurements")
     ap.add_argument("--traj", type=int, default=0, help="trajectories (per target for batch64)")
@@ -425,6 +476,9 @@ def main():
                     help="headline workloads: also build the far field from the trained network "
                          "and report its CCS error")
     ap.add_argument("--surrogate", action="store_true", help="batch64: range-split far fields")
+    ap.add_argument("--epochs", type=int, default=300, help="network: training epochs")
+    ap.add_argument("--sur-lanes", type=int, default=64, choices=[16, 32, 64],
+                    help="headline workloads: lanes per trajectory of the range-split leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB float32 tensor three times: the calibration of "
                          "the FETCH_SIZE / WRITE_SIZE counters for the traffic figure")
@@ -445,6 +499,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch64":
         out = run_batch(args)
+    elif args.workload == "network":
+        out = run_network(args)
     elif args.workload in ("ellipsoid50k", "shell1m"):
         out = run_headline(args, ellipsoid50k if args.workload == "ellipsoid50k" else shell1m)
     else:

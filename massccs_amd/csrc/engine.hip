@@ -102,17 +102,21 @@ EvalFn pick_eval_c(bool cut, bool lds) {
 }
 
 // The site roles whose terms the range-split far field covers (N2 nitrogen
-// and charge/dipole sites, the one-site dipole probe), with cutoffs.
-EvalFn select_eval_split(bool lj, bool coul, bool dip, bool lds) {
-  if (lj && coul && !dip)
-    return lds ? eval_kernel<OneSite<true, false, true, false, true>, 64, true, true>
-               : eval_kernel<OneSite<true, false, true, false, true>, 64, false, true>;
-  if (!lj && coul && dip)
-    return lds ? eval_kernel<OneSite<false, false, true, true, true>, 64, true, true>
-               : eval_kernel<OneSite<false, false, true, true, true>, 64, false, true>;
-  if (lj && !coul && dip)
-    return lds ? eval_kernel<OneSite<true, false, false, true, true>, 64, true, true>
-               : eval_kernel<OneSite<true, false, false, true, true>, 64, false, true>;
+// and charge/dipole sites, the one-site dipole probe), with cutoffs, at L
+// lanes per point (the trajectory kernel's group size: the far field's
+// nodes are split over the lanes as there).
+template <class S, int L>
+EvalFn pick_eval_split(bool lds) {
+  return lds ? eval_kernel<S, L, true, true> : eval_kernel<S, L, false, true>;
+}
+template <class S>
+EvalFn pick_eval_split_l(int L, bool lds) {
+  return L == 16 ? pick_eval_split<S, 16>(lds) : (L == 32 ? pick_eval_split<S, 32>(lds) : pick_eval_split<S, 64>(lds));
+}
+EvalFn select_eval_split(bool lj, bool coul, bool dip, bool lds, int L) {
+  if (lj && coul && !dip) return pick_eval_split_l<OneSite<true, false, true, false, true>>(L, lds);
+  if (!lj && coul && dip) return pick_eval_split_l<OneSite<false, false, true, true, true>>(L, lds);
+  if (lj && !coul && dip) return pick_eval_split_l<OneSite<true, false, false, true, true>>(L, lds);
   return nullptr;
 }
 
@@ -680,7 +684,7 @@ static EvalFn site_kernel(const Engine *E, int site, double &q) {
     throw Error(MCCS_ERR_INVALID, "unknown site role");
   }
   if (!(lj || coul || dip)) return nullptr;  // N2 dummy of force types 1/2: no interaction
-  EvalFn fn = E->split ? select_eval_split(lj, coul, dip, E->lds)
+  EvalFn fn = E->split ? select_eval_split(lj, coul, dip, E->lds, E->lanes)
                        : select_eval(lj, cen, coul, dip, cut, E->lds);
   if (!fn) throw Error(MCCS_ERR_UNSUPPORTED, "no site kernel for this role");
   return fn;
@@ -696,9 +700,10 @@ static void eval_sites_dev(Engine *E, int site, const double *dx, uint64_t n, do
     return;
   }
   const int block = 256;
-  const size_t shmem = lds_bytes(E->T.natoms, E->T.list_cap, block, 64, E->lds, E->gk == 3);
+  const int L = E->split ? E->lanes : 64;  // lanes per point
+  const size_t shmem = lds_bytes(E->T.natoms, E->T.list_cap, block, L, E->lds, E->gk == 3);
   set_shmem((const void *)fn, shmem);
-  const uint64_t gpb = block / 64;
+  const uint64_t gpb = block / L;
   const uint64_t blocks = std::min<uint64_t>((n + gpb - 1) / gpb, (uint64_t)E->cus * 8);
   Tgt T = E->split ? E->Ts : E->T;  // with the far field when it is in use
   void *args[] = {&T, &dx, &n, &q, &df, &du};
@@ -1285,8 +1290,6 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
 void engine_use_surrogate(Engine *E, bool on) {
   HIP_OK(hipSetDevice(E->device));
   if (on && !E->sur_built) throw Error(MCCS_ERR_INVALID, "no surrogate has been built for this engine");
-  if (on && E->lanes != 64)
-    throw Error(MCCS_ERR_INVALID, "the far-field surrogate runs 64 lanes per trajectory");
   E->split = on;
   HIP_OK(hipMemcpy(E->d_tg, on ? &E->Ts : &E->T, sizeof(Tgt), hipMemcpyHostToDevice));
 }
@@ -1375,7 +1378,6 @@ static std::string class_signature(const Batch &B, const BatchClass &c) {
 static void configure_class(Batch &B, BatchClass &c) {
   c.split = B.split;
   for (int t : c.members) c.split = c.split && B.eng[t]->sur_built;
-  if (c.split) c.lanes = 64;  // the far-field kernels run whole wavefronts
   c.fn = c.gk == 1 ? select_traj_multi1(c.ft, c.lanes, B.count, c.split)
                    : (c.gk == 2 ? select_traj_multi2(c.ft, c.lanes, B.count, c.split) : nullptr);
   if (!c.fn)
@@ -1644,7 +1646,6 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
     }();
     for (int L : lanes_c)
       for (double sk : skin_c) {
-        if (split && L != 64) continue;  // the far-field kernels run whole wavefronts
         if (B->lanes_opt && L != B->lanes_opt) continue;
         if (B->skin_opt > 0.0 && sk != B->skin_opt) continue;
         cand.push_back({L, sk});

@@ -756,54 +756,69 @@ __device__ __forceinline__ void bspline_piece(int i, double t, double &w, double
   ddw = i == 0 ? e0 : (i == 1 ? e1 : (i == 2 ? e2 : e3));
 }
 
-// The same in FP32 (the far field's interpolation, whose own error is
-// ~1e-3 of the field, runs in single precision).
-__device__ __forceinline__ void bspline_piece_f(int i, float t, float &w, float &dw, float &ddw) {
-  const float u = 1.0f - t, t2 = t * t, t3 = t2 * t;
-  const float w0 = u * u * u * (1.0f / 6.0f), w1 = (3.0f * t3 - 6.0f * t2 + 4.0f) * (1.0f / 6.0f);
-  const float w2 = (-3.0f * t3 + 3.0f * t2 + 3.0f * t + 1.0f) * (1.0f / 6.0f), w3 = t3 * (1.0f / 6.0f);
-  const float d0 = -0.5f * u * u, d1 = 1.5f * t2 - 2.0f * t, d2 = -1.5f * t2 + t + 0.5f, d3 = 0.5f * t2;
-  const float e0 = u, e1 = 3.0f * t - 2.0f, e2 = 1.0f - 3.0f * t, e3 = t;
-  w = i == 0 ? w0 : (i == 1 ? w1 : (i == 2 ? w2 : w3));
-  dw = i == 0 ? d0 : (i == 1 ? d1 : (i == 2 ? d2 : d3));
-  ddw = i == 0 ? e0 : (i == 1 ? e1 : (i == 2 ? e2 : e3));
+// Piece i (0..3) of the uniform cubic B-spline in FP32 (the far field's
+// interpolation, whose own error is ~1e-3 of the field, runs in single
+// precision), as the power-basis coefficients w(t) = c0 + c1 t + c2 t^2 +
+// c3 t^3: (1 - t)^3 / 6, (3 t^3 - 6 t^2 + 4) / 6, (-3 t^3 + 3 t^2 + 3 t + 1) / 6,
+// t^3 / 6.  A lane's piece index is fixed, so the coefficients are selected
+// once per evaluation and each site costs six FMAs per axis.
+struct BPiece {
+  float c0, c1, c2, c3;
+};
+__device__ __forceinline__ BPiece bspline_coef_f(int i) {
+  BPiece b;
+  b.c0 = i == 0 ? 1.0f / 6.0f : (i == 1 ? 4.0f / 6.0f : (i == 2 ? 1.0f / 6.0f : 0.0f));
+  b.c1 = i == 0 ? -0.5f : (i == 2 ? 0.5f : 0.0f);
+  b.c2 = i == 0 ? 0.5f : (i == 1 ? -1.0f : (i == 2 ? 0.5f : 0.0f));
+  b.c3 = i == 0 ? -1.0f / 6.0f : (i == 1 ? 0.5f : (i == 2 ? -0.5f : 1.0f / 6.0f));
+  return b;
+}
+// w(t), w'(t) and w''(t) of a piece.
+__device__ __forceinline__ void bspline_eval_f(const BPiece &b, float t, float &w, float &dw, float &ddw) {
+  w = fmaf(fmaf(fmaf(b.c3, t, b.c2), t, b.c1), t, b.c0);
+  dw = fmaf(fmaf(3.0f * b.c3, t, 2.0f * b.c2), t, b.c1);
+  ddw = fmaf(6.0f * b.c3, t, 2.0f * b.c2);
 }
 
-// Far part of the range-split field at the sites: every lane of a 64-lane
-// group takes one of the 4 x 4 x 4 B-spline nodes around each site.  The
+// Far part of the range-split field at the sites: the L lanes of a group
+// take the 4 x 4 x 4 B-spline nodes around each site, 64 / L nodes per lane
+// (node k = lane + L h: x offset k / 16, y (k / 4) % 4, z k % 4).  The
 // coefficients are loaded first (far_load, issued before the pair loop so
 // that the loads overlap it) and applied after it (far_apply): each lane adds
-// its node's share of psi and its derivatives to the site's accumulators,
+// its nodes' share of psi and its derivatives to the site's accumulators,
 // which the group reduction then sums together with the near terms.  A
 // charged site gets q K psi and -q K grad psi, a polarisable one
 // E -= grad psi and grad E -= Hess psi.  Nodes outside the grid hold zero
 // (the field vanishes there).
+template <int NPL>
 struct FarNode {
-  float t[3];   // position within the cell of the site, in grid units
-  float c;      // this lane's coefficient
+  float t[3];     // position within the cell of the site, in grid units
+  float c[NPL];   // this lane's coefficients
   bool on;
 };
 
 // The coefficients a trajectory read at its last evaluation, by site: a probe
 // moves about a tenth of a grid spacing per step, so most steps need the same
 // 4 x 4 x 4 nodes again.  b[s][0] = INT_MIN marks an empty entry.
-template <int NS>
+template <int NS, int NPL>
 struct FarCache {
   int b[NS][3];
-  float c[NS];
+  float c[NS][NPL];
   __device__ void clear() {
     for (int s = 0; s < NS; s++) b[s][0] = INT_MIN;
   }
 };
 
-template <class S>
+template <class S, int L>
 __device__ __forceinline__ void far_load(const Tgt &T, int lane, const double (&p)[S::NS][3],
-                                         FarNode (&fn)[S::NS], int only,
-                                         FarCache<S::NS> *cache = nullptr) {
+                                         FarNode<64 / L> (&fn)[S::NS], int only,
+                                         FarCache<S::NS, 64 / L> *cache = nullptr) {
+  constexpr int NPL = 64 / L;
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
     fn[s].on = false;
-    fn[s].c = 0.0f;
+#pragma unroll
+    for (int h = 0; h < NPL; h++) fn[s].c[h] = 0.0f;
     if (!(S::coul(s) || S::dip(s))) continue;
     if (only >= 0 && s != only) continue;
     const double ux = (p[s][0] - T.pox) * T.pih, uy = (p[s][1] - T.poy) * T.pih,
@@ -816,54 +831,69 @@ __device__ __forceinline__ void far_load(const Tgt &T, int lane, const double (&
     fn[s].t[2] = (float)(uz - fz);
     const int bx = (int)fx - 1, by = (int)fy - 1, bz = (int)fz - 1;
     if (cache && cache->b[s][0] == bx && cache->b[s][1] == by && cache->b[s][2] == bz) {
-      fn[s].c = cache->c[s];
+#pragma unroll
+      for (int h = 0; h < NPL; h++) fn[s].c[h] = cache->c[s][h];
       continue;
     }
-    const long ix = (long)bx + (lane >> 4), iy = (long)by + ((lane >> 2) & 3), iz = (long)bz + (lane & 3);
-    if (ix >= 0 && ix < T.pnx && iy >= 0 && iy < T.pny && iz >= 0 && iz < T.pnz)
-      fn[s].c = T.psi[((size_t)ix * T.pny + (size_t)iy) * T.pnz + (size_t)iz];
+    const long iy = (long)by + ((lane >> 2) & 3), iz = (long)bz + (lane & 3);
+#pragma unroll
+    for (int h = 0; h < NPL; h++) {
+      const long ix = (long)bx + ((lane + L * h) >> 4);
+      if (ix >= 0 && ix < T.pnx && iy >= 0 && iy < T.pny && iz >= 0 && iz < T.pnz)
+        fn[s].c[h] = T.psi[((size_t)ix * T.pny + (size_t)iy) * T.pnz + (size_t)iz];
+    }
     if (cache) {
       cache->b[s][0] = bx;
       cache->b[s][1] = by;
       cache->b[s][2] = bz;
-      cache->c[s] = fn[s].c;
+#pragma unroll
+      for (int h = 0; h < NPL; h++) cache->c[s][h] = fn[s].c[h];
     }
   }
 }
 
-template <class S>
-__device__ __forceinline__ void far_apply(const Tgt &T, int lane, const FarNode (&fn)[S::NS],
+template <class S, int L>
+__device__ __forceinline__ void far_apply(const Tgt &T, int lane, const FarNode<64 / L> (&fn)[S::NS],
                                           const double (&qs)[S::NS], Acc (&a)[S::NS]) {
+  constexpr int NPL = 64 / L;
   const float h1 = (float)T.pih, h2 = h1 * h1;
+  BPiece bx[NPL];
+#pragma unroll
+  for (int h = 0; h < NPL; h++) bx[h] = bspline_coef_f((lane + L * h) >> 4);
+  const BPiece by = bspline_coef_f((lane >> 2) & 3), bz = bspline_coef_f(lane & 3);
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
     if (!(S::coul(s) || S::dip(s))) continue;
     if (!fn[s].on) continue;
-    const float c = fn[s].c;
-    float wx, dwx, ddwx, wy, dwy, ddwy, wz, dwz, ddwz;
-    bspline_piece_f(lane >> 4, fn[s].t[0], wx, dwx, ddwx);
-    bspline_piece_f((lane >> 2) & 3, fn[s].t[1], wy, dwy, ddwy);
-    bspline_piece_f(lane & 3, fn[s].t[2], wz, dwz, ddwz);
-    const float cyz = c * wy * wz, cxz = c * wx * wz, cxy = c * wx * wy;
-    const double gx = (double)(dwx * cyz * h1), gy = (double)(dwy * cxz * h1),
-                 gz = (double)(dwz * cxy * h1);
-    if (S::coul(s)) {
-      const double qk = qs[s] * KC;
-      a[s].U += qk * (double)(wx * cyz);
-      a[s].fx -= qk * gx;
-      a[s].fy -= qk * gy;
-      a[s].fz -= qk * gz;
-    }
-    if (S::dip(s)) {
-      a[s].Ex -= gx;
-      a[s].Ey -= gy;
-      a[s].Ez -= gz;
-      a[s].Exx -= (double)(ddwx * cyz * h2);
-      a[s].Eyy -= (double)(ddwy * cxz * h2);
-      a[s].Ezz -= (double)(ddwz * cxy * h2);
-      a[s].Exy -= (double)(dwx * dwy * (c * wz) * h2);
-      a[s].Exz -= (double)(dwx * dwz * (c * wy) * h2);
-      a[s].Eyz -= (double)(dwy * dwz * (c * wx) * h2);
+    float wy, dwy, ddwy, wz, dwz, ddwz;
+    bspline_eval_f(by, fn[s].t[1], wy, dwy, ddwy);
+    bspline_eval_f(bz, fn[s].t[2], wz, dwz, ddwz);
+#pragma unroll
+    for (int h = 0; h < NPL; h++) {
+      const float c = fn[s].c[h];
+      float wx, dwx, ddwx;
+      bspline_eval_f(bx[h], fn[s].t[0], wx, dwx, ddwx);
+      const float cyz = c * wy * wz, cxz = c * wx * wz, cxy = c * wx * wy;
+      const double gx = (double)(dwx * cyz * h1), gy = (double)(dwy * cxz * h1),
+                   gz = (double)(dwz * cxy * h1);
+      if (S::coul(s)) {
+        const double qk = qs[s] * KC;
+        a[s].U += qk * (double)(wx * cyz);
+        a[s].fx -= qk * gx;
+        a[s].fy -= qk * gy;
+        a[s].fz -= qk * gz;
+      }
+      if (S::dip(s)) {
+        a[s].Ex -= gx;
+        a[s].Ey -= gy;
+        a[s].Ez -= gz;
+        a[s].Exx -= (double)(ddwx * cyz * h2);
+        a[s].Eyy -= (double)(ddwy * cxz * h2);
+        a[s].Ezz -= (double)(ddwz * cxy * h2);
+        a[s].Exy -= (double)(dwx * dwy * (c * wz) * h2);
+        a[s].Exz -= (double)(dwx * dwz * (c * wy) * h2);
+        a[s].Eyz -= (double)(dwy * dwz * (c * wx) * h2);
+      }
     }
   }
 }
@@ -872,8 +902,7 @@ __device__ __forceinline__ void far_apply(const Tgt &T, int lane, const FarNode 
 template <class S, int L, bool LDS, bool COUNT, bool SPLIT = false>
 __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p)[S::NS][3],
                          const double (&qs)[S::NS], double (&f)[S::NS][3], double (&U)[S::NS],
-                         Work &w, VList *vl, FarCache<S::NS> *fcache = nullptr) {
-  static_assert(!SPLIT || L == 64, "the far field is evaluated by whole wavefronts");
+                         Work &w, VList *vl, FarCache<S::NS, 64 / L> *fcache = nullptr) {
   constexpr bool CEN = any_central<S>();
 #pragma unroll
   for (int s = 0; s < S::NS; s++) {
@@ -953,8 +982,8 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
         double2 lp = make_double2(0.0, 0.0), lc = make_double2(0.0, 0.0);
         // the far field's coefficients are loaded before the pair loop (their
         // latency overlaps it) and applied after it
-        FarNode fn[S::NS];
-        if constexpr (SPLIT) far_load<S>(T, lane, p, fn, -1, fcache);
+        FarNode<64 / L> fn[S::NS];
+        if constexpr (SPLIT) far_load<S, L>(T, lane, p, fn, -1, fcache);
         if (lane < n) fetch<LDS, CEN>(T, sh, entry(lane), at, lp, lc);
         for (int t = lane; t < n; t += L) {
           const double4 cat = at;
@@ -962,7 +991,7 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
           if (t + L < n) fetch<LDS, CEN>(T, sh, entry(t + L), at, lp, lc);
           pair_terms<S, COUNT, L == 64, SPLIT>(T, p, qs, cat, clp, clc, t < nin, a, w);
         }
-        if constexpr (SPLIT) far_apply<S>(T, lane, fn, qs, a);
+        if constexpr (SPLIT) far_apply<S, L>(T, lane, fn, qs, a);
         reduce<S, L>(a, sh.red);
         finish<S>(T, a, -1, f, U);
         return;
@@ -984,9 +1013,9 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     g.nys = iy1 - iy0 + 1;
     walk_runs<L>(T, sh, lane, g, (ix1 - ix0 + 1) * g.nys, visit);
     if constexpr (SPLIT) {
-      FarNode fn[S::NS];
-      far_load<S>(T, lane, p, fn, -1);
-      far_apply<S>(T, lane, fn, qs, a);
+      FarNode<64 / L> fn[S::NS];
+      far_load<S, L>(T, lane, p, fn, -1);
+      far_apply<S, L>(T, lane, fn, qs, a);
     }
     reduce<S, L>(a, sh.red);
     finish<S>(T, a, -1, f, U);
@@ -1024,9 +1053,9 @@ __device__ void evaluate(const Tgt &T, const Shm &sh, int lane, const double (&p
     };
     walk_runs<L>(T, sh, lane, g, nr, visit1);
     if constexpr (SPLIT) {
-      FarNode fn[S::NS];
-      far_load<S>(T, lane, pp, fn, ps);
-      far_apply<S>(T, lane, fn, qs, b);
+      FarNode<64 / L> fn[S::NS];
+      far_load<S, L>(T, lane, pp, fn, ps);
+      far_apply<S, L>(T, lane, fn, qs, b);
     }
     reduce<S, L>(b, sh.red);
     finish<S>(T, b, ps, f, U);
@@ -1526,7 +1555,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
   Work w{0, 0, 0, 0};
   int status = ST_NEED;
   unsigned long long wait_since = 0;  // wall clock when the group began waiting, 0 if not
-  FarCache<S::NS> fcache;  // far-field coefficients of the last evaluation (range split)
+  FarCache<S::NS, 64 / L> fcache;  // far-field coefficients of the last evaluation (range split)
   fcache.clear();
   if (lane == 0) {
     st.vl.valid = -1;
@@ -1990,16 +2019,26 @@ TrajFn pick_traj_full(int L, bool lds, bool count) {
 // N2 type 6): 64 lanes.
 template <int GK, int FT>
 TrajFn pick_traj_split(int L, bool lds, bool count) {
-  if (L != 64) return nullptr;
-  if (lds)
-    return count ? traj_kernel<GK, FT, 64, true, true, false, true> : traj_kernel<GK, FT, 64, true, false, false, true>;
-  return count ? traj_kernel<GK, FT, 64, false, true, false, true> : traj_kernel<GK, FT, 64, false, false, false, true>;
+  switch (L) {
+    case 16:
+      if (lds) return count ? traj_kernel<GK, FT, 16, true, true, false, true> : traj_kernel<GK, FT, 16, true, false, false, true>;
+      return count ? traj_kernel<GK, FT, 16, false, true, false, true> : traj_kernel<GK, FT, 16, false, false, false, true>;
+    case 32:
+      if (lds) return count ? traj_kernel<GK, FT, 32, true, true, false, true> : traj_kernel<GK, FT, 32, true, false, false, true>;
+      return count ? traj_kernel<GK, FT, 32, false, true, false, true> : traj_kernel<GK, FT, 32, false, false, false, true>;
+    default:
+      if (lds) return count ? traj_kernel<GK, FT, 64, true, true, false, true> : traj_kernel<GK, FT, 64, true, false, false, true>;
+      return count ? traj_kernel<GK, FT, 64, false, true, false, true> : traj_kernel<GK, FT, 64, false, false, false, true>;
+  }
 }
 
 template <int GK, int FT>
 TrajFn pick_traj_multi_split(int L, bool count) {
-  if (L != 64) return nullptr;
-  return count ? traj_kernel<GK, FT, 64, false, true, true, true> : traj_kernel<GK, FT, 64, false, false, true, true>;
+  switch (L) {
+    case 16: return count ? traj_kernel<GK, FT, 16, false, true, true, true> : traj_kernel<GK, FT, 16, false, false, true, true>;
+    case 32: return count ? traj_kernel<GK, FT, 32, false, true, true, true> : traj_kernel<GK, FT, 32, false, false, true, true>;
+    default: return count ? traj_kernel<GK, FT, 64, false, true, true, true> : traj_kernel<GK, FT, 64, false, false, true, true>;
+  }
 }
 
 template <int GK, int FT>

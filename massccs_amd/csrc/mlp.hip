@@ -1,14 +1,12 @@
 // Fourier-feature MLP of the far-field surrogate: BF16 MFMA forward,
 // backward and Adam kernels with FP32 accumulation (gfx950).
 //
-// Matrix products use v_mfma_f32_16x16x16_bf16: one wavefront computes a
-// 32 x 32 output tile as 2 x 2 MFMA tiles; a block of four wavefronts covers
-// 64 x 64.  Operand fragments (lane l): A[m = l % 16][k = 4 (l / 16) + e],
-// B[k = 4 (l / 16) + e][n = l % 16], e = 0..3; result D[m = 4 (l / 16) + e]
-// [n = l % 16].  Products whose reduction runs over the batch (weight
-// gradients) are split along k over blocks; each block writes its partial
-// product and the partials are summed in k order, so that training gives the
-// same weights on every run.
+// Matrix products run on v_mfma_f32_16x16x32_bf16 over LDS-staged tiles
+// (gemm_kernel), with the bias + SiLU of a forward layer and the SiLU'
+// factor of a backward layer applied in the product's epilogue.  Products
+// whose reduction runs over the batch (weight gradients) are split along k
+// over blocks; each block writes its partial product and the partials are
+// summed in k order, so that training gives the same weights on every run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,7 +31,6 @@
 namespace mccs {
 namespace mlpdev {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr float TWO_PI = 6.283185307179586f;
@@ -70,17 +67,92 @@ __global__ void features_kernel(const float *__restrict__ xyz, int n, DSpec sp,
   X0[(size_t)p * MLP_IN + MLP_NF + f] = to_bf16(cv);
 }
 
-// C (M x N, FP32) = op(A) (M x K) * op(B) (K x N), BF16 operands.
+// C (M x N, FP32) = op(A) (M x K) * op(B) (K x N), BF16 operands, FP32 sums.
 // op(A)[m][k] = TA ? A[k lda + m] : A[m lda + k]; op(B)[k][n] = TB ? B[n ldb + k] : B[k ldb + n].
-// blockIdx.z selects a k-chunk of kchunk (multiple of 16); SPLIT writes the
-// chunk's product to slice z of C (slices of M ldc floats).
-template <bool TA, bool TB, bool SPLIT>
+//
+// A block of four wavefronts computes a GT x GT tile of C; each wavefront a
+// 32 x 32 quarter as 2 x 2 tiles of v_mfma_f32_16x16x32_bf16.  The k range
+// is staged through LDS GK at a time, both operands stored k-contiguous
+// (As[m][k], Bs[n][k], rows padded to GP elements so that the 16 rows one
+// fragment read touches start in distinct bank groups); every global load
+// is one 16-byte vector of 8 BF16 along the operand's contiguous axis, and
+// the next stage's loads are in flight while the current stage multiplies.
+// The MFMA computes the transposed tile (B^T A^T) so that each lane ends
+// with four consecutive columns of one row and stores them as one vector.
+// Fragments (lane l, q = l / 16): first operand B[k = 8 q + e][n = l % 16],
+// second A[m = l % 16][k = 8 q + e], e = 0..7; result C[m = l % 16][n = 4 q
+// + e], e = 0..3.
+//
+// blockIdx.z selects a k-chunk of kchunk (a multiple of GK); SPLIT writes the
+// chunk's product to slice z of C (slices of M ldc floats).  Epilogues:
+// EPI_STORE C = product; EPI_BIAS_SILU (forward layer) C = Z = product +
+// bias[n] and H = bf16(SiLU(Z)); EPI_SILU_BACK (backward layer) H = bf16(dZ),
+// dZ = product * SiLU'(Zin), and C receives the block's column sums of dZ
+// over its GT rows: C[blockIdx.y][n] (the partials of the bias gradient).
+constexpr int GT = 64, GK = 64, GP = GK + 8;
+enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_SILU_BACK = 2 };
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+// One 8-element chunk of an operand tile, read along the operand's
+// contiguous axis: rows [row0, row0 + rows) x contiguous [c0, c0 + 8) of a
+// matrix with leading dimension ld whose valid extent is (row_end, c_end).
+__device__ __forceinline__ u16x8 load_chunk(const uint16_t *__restrict__ X, int ld, int row, int row_end,
+                                            int c, int c_end) {
+  u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (row >= row_end) return v;
+  const uint16_t *p = X + (size_t)row * ld + c;
+  if (c + 8 <= c_end) return *reinterpret_cast<const u16x8 *>(p);
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+    if (c + e < c_end) v[e] = p[e];
+  return v;
+}
+
+// The GT x GK tile of an operand at (outer0, k0) into registers (two chunks
+// per thread).  KC: the operand is k-contiguous in memory (rows = outer
+// index); otherwise outer-contiguous (rows = k).
+template <bool KC>
+__device__ __forceinline__ void load_tile(const uint16_t *__restrict__ X, int ld, int outer0, int outer_end,
+                                          int k0, int k_end, u16x8 r[2]) {
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int c = threadIdx.x + 256 * h;
+    const int row = c >> 3, col = (c & 7) * 8;
+    if (KC) r[h] = load_chunk(X, ld, outer0 + row, outer_end, k0 + col, k_end);
+    else r[h] = load_chunk(X, ld, k0 + row, k_end, outer0 + col, outer_end);
+  }
+}
+
+// Registers of load_tile into the k-contiguous LDS tile S[outer][k].
+template <bool KC>
+__device__ __forceinline__ void store_tile(uint16_t *S, const u16x8 r[2]) {
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int c = threadIdx.x + 256 * h;
+    const int row = c >> 3, col = (c & 7) * 8;
+    if (KC) {
+      *reinterpret_cast<u16x8 *>(S + row * GP + col) = r[h];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) S[(col + e) * GP + row] = r[h][e];
+    }
+  }
+}
+
+template <bool TA, bool TB, bool SPLIT, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const uint16_t *__restrict__ A,
                                                    int lda, const uint16_t *__restrict__ B, int ldb,
-                                                   float *__restrict__ C, int ldc, int kchunk) {
+                                                   float *__restrict__ C, int ldc, int kchunk,
+                                                   const float *__restrict__ bias,
+                                                   const float *__restrict__ Zin,
+                                                   uint16_t *__restrict__ H) {
+  __shared__ __align__(16) uint16_t As[GT * GP];
+  __shared__ __align__(16) uint16_t Bs[GT * GP];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int m0 = blockIdx.y * 64 + (w / 2) * 32;
-  const int n0 = blockIdx.x * 64 + (w % 2) * 32;
+  const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
+  const int wm = (w / 2) * 32, wn = (w % 2) * 32;
   const int kb = blockIdx.z * kchunk;
   const int ke = min(K, kb + kchunk);
   const int r16 = lane % 16, q = lane / 16;
@@ -90,65 +162,152 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k = kb; k < ke; k += 16) {
-    s4 a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      const int m = m0 + 16 * i + r16;
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int kk = k + 4 * q + e;
-        uint16_t v = 0;
-        if (m < M && kk < ke) v = TA ? A[(size_t)kk * lda + m] : A[(size_t)m * lda + kk];
-        a[i][e] = (short)v;
-      }
+  u16x8 ra[2], rb[2];
+  load_tile<!TA>(A, lda, bm, M, kb, ke, ra);
+  load_tile<TB>(B, ldb, bn, N, kb, ke, rb);
+  for (int k0 = kb; k0 < ke; k0 += GK) {
+    __syncthreads();  // the previous stage's fragments are read
+    store_tile<!TA>(As, ra);
+    store_tile<TB>(Bs, rb);
+    __syncthreads();
+    if (k0 + GK < ke) {
+      load_tile<!TA>(A, lda, bm, M, k0 + GK, ke, ra);
+      load_tile<TB>(B, ldb, bn, N, k0 + GK, ke, rb);
     }
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int nn = n0 + 16 * j + r16;
+    for (int ks = 0; ks < GK; ks += 32) {
+      bf8 a[2], b[2];
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int kk = k + 4 * q + e;
-        uint16_t v = 0;
-        if (nn < N && kk < ke) v = TB ? B[(size_t)nn * ldb + kk] : B[(size_t)kk * ldb + nn];
-        b[j][e] = (short)v;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
+      for (int i = 0; i < 2; i++)
+        a[i] = *reinterpret_cast<const bf8 *>(As + (wm + 16 * i + r16) * GP + ks + 8 * q);
 #pragma unroll
       for (int j = 0; j < 2; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-  }
+        b[j] = *reinterpret_cast<const bf8 *>(Bs + (wn + 16 * j + r16) * GP + ks + 8 * q);
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+    }
+  }
+  if (EPI == EPI_SILU_BACK) {
+    // column sums over the wavefront's 32 rows: the two tiles of a lane,
+    // then the 16 lanes of equal q (rows l % 16), read from lane q * 16
+    float cs[2][4];
 #pragma unroll
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int m = m0 + 16 * i + 4 * q + e;
-        const int nn = n0 + 16 * j + r16;
-        if (m < M && nn < N) C[(size_t)m * ldc + nn] = acc[i][j][e];
+        float v = 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const int m = bm + wm + 16 * i + r16;
+          const int nn = bn + wn + 16 * j + 4 * q + e;
+          float dz = 0.f;
+          if (m < M && nn < N) {
+            const float z = Zin[(size_t)m * ldc + nn], sg = sigm(z);
+            dz = acc[i][j][e] * sg * (1.f + z * (1.f - sg));
+          }
+          acc[i][j][e] = dz;
+          v += dz;
+        }
+        cs[j][e] = v;
       }
+#pragma unroll
+    for (int sh = 1; sh < 16; sh <<= 1)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) cs[j][e] += __shfl_xor(cs[j][e], sh);
+    __shared__ float colp[2][GT];
+    if (r16 == 0)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) colp[w / 2][wn + 16 * j + 4 * q + e] = cs[j][e];
+    __syncthreads();
+    if (threadIdx.x < GT && bn + (int)threadIdx.x < N)
+      C[(size_t)blockIdx.y * ldc + bn + threadIdx.x] = colp[0][threadIdx.x] + colp[1][threadIdx.x];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int m = bm + wm + 16 * i + r16;
+      const int n0 = bn + wn + 16 * j + 4 * q;  // N and ldc are multiples of 4
+      if (m >= M || n0 >= N) continue;
+      const size_t o = (size_t)m * ldc + n0;
+      f4 v = acc[i][j];
+      if (EPI == EPI_BIAS_SILU) {
+        const f4 bb = *reinterpret_cast<const f4 *>(bias + n0);
+        v += bb;
+        *reinterpret_cast<f4 *>(C + o) = v;
+      } else if (EPI == EPI_STORE) {
+        *reinterpret_cast<f4 *>(C + o) = v;
+      }
+      if (EPI != EPI_STORE) {
+        uint16_t hv[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) hv[e] = to_bf16(EPI == EPI_BIAS_SILU ? v[e] * sigm(v[e]) : v[e]);
+        uint2 packed;
+        packed.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
+        packed.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
+        *reinterpret_cast<uint2 *>(H + o) = packed;
+      }
+    }
 }
 
-// out[i] += the sum of part[z][i] over z = 0 .. nz - 1, in that order.
-__global__ void splitsum_kernel(const float *__restrict__ part, int nz, int n, float *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// out[i] += the sum of part[z][i] over z = 0 .. nz - 1 (slices of width
+// floats).  64 threads per output element: thread g sums z = g, g + 64, ...
+// in order, and the 64 partial sums are combined by a fixed tree, so the
+// result is the same on every run.
+__global__ __launch_bounds__(256) void partsum_kernel(const float *__restrict__ part, int nz, int width,
+                                                      float *__restrict__ out) {
+  __shared__ float red[256];
+  const int i = blockIdx.x * 4 + threadIdx.x / 64, g = threadIdx.x % 64;
   float s = 0.f;
-  for (int z = 0; z < nz; z++) s += part[(size_t)z * n + i];
-  out[i] += s;
+  if (i < width)
+    for (int z = g; z < nz; z += 64) s += part[(size_t)z * width + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (g < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (g == 0 && i < width) out[i] += red[threadIdx.x];
 }
 
-// Z += bias; H = bf16(SiLU(Z)).
-__global__ void bias_silu_kernel(float *__restrict__ Z, const float *__restrict__ bias, int n,
-                                 uint16_t *__restrict__ H) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)n * MLP_H) return;
-  const float z = Z[i] + bias[i % MLP_H];
-  Z[i] = z;
-  H[i] = to_bf16(z * sigm(z));
+// Per-block partial gradients of the output layer over rows [b rpb, (b + 1)
+// rpb): part[b][2 j + o] = sum_p H3[p][j] dY[p][o], part[b][2 MLP_H + o] =
+// sum_p dY[p][o] (the bias).
+constexpr int W4PART = 2 * MLP_H + 2;
+__global__ __launch_bounds__(256) void w4part_kernel(const uint16_t *__restrict__ H, const float *__restrict__ dY,
+                                                     int n, int rpb, float *__restrict__ part) {
+  __shared__ float red[256][3];
+  const int j = threadIdx.x % MLP_H, h = threadIdx.x / MLP_H;
+  const int r0 = blockIdx.x * rpb, r1 = min(n, r0 + rpb);
+  float s0 = 0.f, s1 = 0.f, sb0 = 0.f, sb1 = 0.f;
+  for (int r = r0 + h; r < r1; r += 2) {
+    const float hv = from_bf16(H[(size_t)r * MLP_H + j]);
+    const float d0 = dY[2 * r], d1 = dY[2 * r + 1];
+    s0 += hv * d0;
+    s1 += hv * d1;
+    sb0 += d0;
+    sb1 += d1;
+  }
+  red[threadIdx.x][0] = s0;
+  red[threadIdx.x][1] = s1;
+  red[threadIdx.x][2] = j == 0 ? sb0 : sb1;
+  __syncthreads();
+  if (h == 0) {
+    float *o = part + (size_t)blockIdx.x * W4PART;
+    o[2 * j] = red[j][0] + red[MLP_H + j][0];
+    o[2 * j + 1] = red[j][1] + red[MLP_H + j][1];
+    if (j < 2) {
+      // thread j == 0 summed dY[.][0] in slot 2, thread 1 dY[.][1]
+      o[2 * MLP_H + j] = red[j][2] + red[MLP_H + j][2];
+    }
+  }
 }
 
 // Y = H3 W4 + b4 (BF16 operands, FP32 sums).
@@ -197,77 +356,29 @@ __global__ void seed_kernel(int n, int o, float *__restrict__ dY) {
   dY[2 * p + 1] = o == 1 ? 1.f : 0.f;
 }
 
-// dZ3 = (dY W4^T) * SiLU'(Z3), FP32 and BF16 copies.
-__global__ void out_back_kernel(const float *__restrict__ dY, const uint16_t *__restrict__ W4,
-                                const float *__restrict__ Z, int n, float *__restrict__ dZf,
-                                uint16_t *__restrict__ dZb) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)n * MLP_H) return;
-  const size_t p = i / MLP_H;
-  const int j = (int)(i % MLP_H);
-  const float dh = dY[2 * p] * from_bf16(W4[2 * j]) + dY[2 * p + 1] * from_bf16(W4[2 * j + 1]);
-  const float z = Z[i], s = sigm(z);
-  const float dz = dh * s * (1.f + z * (1.f - s));
-  dZf[i] = dz;
-  dZb[i] = to_bf16(dz);
-}
-
-// dZ = dH * SiLU'(Z).
-__global__ void silu_back_kernel(const float *__restrict__ dH, const float *__restrict__ Z, int n,
-                                 float *__restrict__ dZf, uint16_t *__restrict__ dZb) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)n * MLP_H) return;
-  const float z = Z[i], s = sigm(z);
-  const float dz = dH[i] * s * (1.f + z * (1.f - s));
-  dZf[i] = dz;
-  dZb[i] = to_bf16(dz);
-}
-
-// out[c] += sum over rows of A[r][c] (block per column, rows strided).
-__global__ void colsum_kernel(const float *__restrict__ A, int n, int ncol, float *__restrict__ out) {
+// dZ3 = (dY W4^T) * SiLU'(Z3) as BF16 over rows [b rpb, (b + 1) rpb), and
+// the block's column sums of dZ3 (the partials of the bias gradient):
+// part[b][j].  Two rows per step, each handled by 128 consecutive threads.
+__global__ __launch_bounds__(256) void out_back_kernel(const float *__restrict__ dY,
+                                                       const uint16_t *__restrict__ W4,
+                                                       const float *__restrict__ Z, int n, int rpb,
+                                                       uint16_t *__restrict__ dZb, float *__restrict__ part) {
   __shared__ float red[256];
-  const int c = blockIdx.x;
-  float s = 0.f;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) s += A[(size_t)r * ncol + c];
-  red[threadIdx.x] = s;
+  const int j = threadIdx.x % MLP_H, h = threadIdx.x / MLP_H;
+  const int r0 = blockIdx.x * rpb, r1 = min(n, r0 + rpb);
+  const float w0 = from_bf16(W4[2 * j]), w1 = from_bf16(W4[2 * j + 1]);
+  float cs = 0.f;
+  for (int r = r0 + h; r < r1; r += 2) {
+    const size_t i = (size_t)r * MLP_H + j;
+    const float dh = dY[2 * r] * w0 + dY[2 * r + 1] * w1;
+    const float z = Z[i], sg = sigm(z);
+    const float dz = dh * sg * (1.f + z * (1.f - sg));
+    dZb[i] = to_bf16(dz);
+    cs += dz;
+  }
+  red[threadIdx.x] = cs;
   __syncthreads();
-  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(out + c, red[0]);
-}
-
-// gW4[j][o] += sum_p H3[p][j] dY[p][o]; gb4[o] += sum_p dY[p][o].
-__global__ void w4grad_kernel(const uint16_t *__restrict__ H, const float *__restrict__ dY, int n,
-                              float *__restrict__ gW4, float *__restrict__ gb4) {
-  __shared__ float r0[256], r1[256];
-  const int j = blockIdx.x;  // 0..MLP_H (MLP_H: bias)
-  float s0 = 0.f, s1 = 0.f;
-  for (int p = threadIdx.x; p < n; p += blockDim.x) {
-    const float h = j < MLP_H ? from_bf16(H[(size_t)p * MLP_H + j]) : 1.f;
-    s0 += h * dY[2 * p];
-    s1 += h * dY[2 * p + 1];
-  }
-  r0[threadIdx.x] = s0;
-  r1[threadIdx.x] = s1;
-  __syncthreads();
-  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) {
-      r0[threadIdx.x] += r0[threadIdx.x + k];
-      r1[threadIdx.x] += r1[threadIdx.x + k];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (j < MLP_H) {
-      gW4[2 * j] += r0[0];
-      gW4[2 * j + 1] += r1[0];
-    } else {
-      gb4[0] += r0[0];
-      gb4[1] += r1[0];
-    }
-  }
+  if (h == 0) part[(size_t)blockIdx.x * MLP_H + j] = red[j] + red[MLP_H + j];
 }
 
 // dE/dr from dX0 (features) through the Fourier map.
@@ -391,8 +502,8 @@ struct Mlp {
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *wb = nullptr;
   uint16_t *X0 = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *dZb = nullptr;
-  float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *Y = nullptr, *dY = nullptr, *dH = nullptr,
-        *dZf = nullptr, *dX = nullptr, *pts = nullptr, *grd = nullptr;
+  float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *Y = nullptr, *dY = nullptr,
+         *dX = nullptr, *pts = nullptr, *grd = nullptr;
   uint16_t *dZ1b = nullptr, *dZ2b = nullptr, *dZ3b = nullptr;
   int *idx = nullptr;
   double *loss = nullptr;
@@ -405,9 +516,9 @@ struct Mlp {
     return reinterpret_cast<T *>(p);
   }
   ~Mlp() {
-    hipSetDevice(device);
-    for (void *p : allocs) hipFree(p);
-    if (stream) hipStreamDestroy(stream);
+    (void)hipSetDevice(device);
+    for (void *p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -417,27 +528,50 @@ unsigned blocks_for(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t);
 
 constexpr int MAX_SPLIT = 64;  // k-chunks of a split product
 
-// C = op(A) op(B).  Split (part != null): C += op(A) op(B), the k range cut
-// into at most MAX_SPLIT chunks whose products go to part (MAX_SPLIT M N
-// floats) and are summed in chunk order; C then has ldc == N.
-template <bool TA, bool TB>
+// C = op(A) op(B) with epilogue EPI (bias and Zin/H as gemm_kernel takes
+// them).  Split (part != null, EPI_STORE only): C += op(A) op(B), the k range
+// cut into at most MAX_SPLIT chunks whose products go to part (MAX_SPLIT M N
+// floats) and are summed in chunk order; C then has ldc == N.  The operands'
+// leading dimensions and base addresses must allow 16-byte loads.
+template <bool TA, bool TB, int EPI = EPI_STORE>
 void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
-          float *C, int ldc, int splitk = 1, float *part = nullptr) {
+          float *C, int ldc, int splitk = 1, float *part = nullptr, const float *bias = nullptr,
+          const float *Zin = nullptr, uint16_t *H = nullptr) {
+  if (lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15))
+    throw Error(MCCS_ERR_INVALID, "matrix operand not aligned for 16-byte loads");
+  if (N % 4 || ldc % 4 || ((uintptr_t)C & 15) || ((uintptr_t)H & 7))
+    throw Error(MCCS_ERR_INVALID, "matrix result not aligned for vector stores");
+  if (EPI == EPI_SILU_BACK && (size_t)((M + GT - 1) / GT) * ldc > (size_t)MAX_SPLIT * MLP_H * MLP_H)
+    throw Error(MCCS_ERR_INVALID, "column partials outside their workspace");
   int kchunk = (K + splitk - 1) / splitk;
-  kchunk = (kchunk + 15) / 16 * 16;
+  kchunk = (kchunk + GK - 1) / GK * GK;
   splitk = (K + kchunk - 1) / kchunk;
   if (!part) {
-    dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
-    hipLaunchKernelGGL((gemm_kernel<TA, TB, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
-                       C, ldc, K);
+    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, 1);
+    hipLaunchKernelGGL((gemm_kernel<TA, TB, false, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
+                       C, ldc, K, bias, Zin, H);
     return;
   }
-  if (splitk > MAX_SPLIT || ldc != N) throw Error(MCCS_ERR_INVALID, "split product outside its workspace");
-  dim3 grid((N + 63) / 64, (M + 63) / 64, splitk);
-  hipLaunchKernelGGL((gemm_kernel<TA, TB, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
-                     part, N, kchunk);
-  hipLaunchKernelGGL(splitsum_kernel, dim3(blocks_for((size_t)M * N)), dim3(256), 0, s,
-                     (const float *)part, splitk, M * N, C);
+  if (EPI != EPI_STORE || splitk > MAX_SPLIT || ldc != N)
+    throw Error(MCCS_ERR_INVALID, "split product outside its workspace");
+  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, splitk);
+  hipLaunchKernelGGL((gemm_kernel<TA, TB, true, EPI_STORE>), grid, dim3(256), 0, s, M, N, K, A, lda, B,
+                     ldb, part, N, kchunk, (const float *)nullptr, (const float *)nullptr,
+                     (uint16_t *)nullptr);
+  hipLaunchKernelGGL(partsum_kernel, dim3((M * N + 3) / 4), dim3(256), 0, s, (const float *)part, splitk,
+                     M * N, C);
+}
+
+// Row blocks of the column reductions: at least 64 rows each, at most
+// MAX_ROWBLOCKS blocks (their partials fit the split-product workspace).
+constexpr int MAX_ROWBLOCKS = 512;
+int rows_per_block(int n) { return std::max(64, (n + MAX_ROWBLOCKS - 1) / MAX_ROWBLOCKS); }
+static_assert((size_t)MAX_ROWBLOCKS * W4PART <= (size_t)MAX_SPLIT * MLP_H * MLP_H,
+              "column partials exceed the workspace");
+
+// out[c] += the sum over nb row blocks of part[b][c], c < MLP_H, in a fixed order.
+void colsum(hipStream_t s, const float *part, int nb, float *out) {
+  hipLaunchKernelGGL(partsum_kernel, dim3((MLP_H + 3) / 4), dim3(256), 0, s, part, nb, MLP_H, out);
 }
 
 // Forward over n <= maxb points already in m->pts: leaves Z*, H*, Y.
@@ -445,15 +579,12 @@ void forward(Mlp *m, int n) {
   hipStream_t s = m->stream;
   hipLaunchKernelGGL(features_kernel, dim3(blocks_for((size_t)n * MLP_NF)), dim3(256), 0, s,
                      m->pts, n, m->ds, m->X0);
-  gemm<false, false>(s, n, MLP_H, MLP_IN, m->X0, MLP_IN, m->wb + MLP_W1, MLP_H, m->Z1, MLP_H);
-  hipLaunchKernelGGL(bias_silu_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->Z1,
-                     m->w + MLP_B1, n, m->H1);
-  gemm<false, false>(s, n, MLP_H, MLP_H, m->H1, MLP_H, m->wb + MLP_W2, MLP_H, m->Z2, MLP_H);
-  hipLaunchKernelGGL(bias_silu_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->Z2,
-                     m->w + MLP_B2, n, m->H2);
-  gemm<false, false>(s, n, MLP_H, MLP_H, m->H2, MLP_H, m->wb + MLP_W3, MLP_H, m->Z3, MLP_H);
-  hipLaunchKernelGGL(bias_silu_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->Z3,
-                     m->w + MLP_B3, n, m->H3);
+  gemm<false, false, EPI_BIAS_SILU>(s, n, MLP_H, MLP_IN, m->X0, MLP_IN, m->wb + MLP_W1, MLP_H, m->Z1, MLP_H,
+                                    1, nullptr, m->w + MLP_B1, nullptr, m->H1);
+  gemm<false, false, EPI_BIAS_SILU>(s, n, MLP_H, MLP_H, m->H1, MLP_H, m->wb + MLP_W2, MLP_H, m->Z2, MLP_H,
+                                    1, nullptr, m->w + MLP_B2, nullptr, m->H2);
+  gemm<false, false, EPI_BIAS_SILU>(s, n, MLP_H, MLP_H, m->H2, MLP_H, m->wb + MLP_W3, MLP_H, m->Z3, MLP_H,
+                                    1, nullptr, m->w + MLP_B3, nullptr, m->H3);
   hipLaunchKernelGGL(out_layer_kernel, dim3(blocks_for(n)), dim3(256), 0, s, m->H3, m->wb + MLP_W4,
                      m->w + MLP_B4, n, m->Y);
 }
@@ -462,28 +593,34 @@ void forward(Mlp *m, int n) {
 // input-feature gradient into m->dX when xgrad.
 void backward(Mlp *m, int n, bool wgrad, bool xgrad) {
   hipStream_t s = m->stream;
-  const int sk = std::max(1, std::min(64, n / 256));
-  if (wgrad)
-    hipLaunchKernelGGL(w4grad_kernel, dim3(MLP_H + 1), dim3(256), 0, s, m->H3, m->dY, n,
-                       m->g + MLP_W4, m->g + MLP_B4);
-  hipLaunchKernelGGL(out_back_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->dY,
-                     m->wb + MLP_W4, m->Z3, n, m->dZf, m->dZ3b);
+  const int sk = std::max(1, std::min(MAX_SPLIT, n / 256));
   if (wgrad) {
-    hipLaunchKernelGGL(colsum_kernel, dim3(MLP_H), dim3(256), 0, s, m->dZf, n, MLP_H, m->g + MLP_B3);
-    gemm<true, false>(s, MLP_H, MLP_H, n, m->H2, MLP_H, m->dZ3b, MLP_H, m->g + MLP_W3, MLP_H, sk, m->part);
+    // gW4 and gb4 are adjacent in the flat layout: one fixed-order reduction
+    static_assert(MLP_B4 == MLP_W4 + 2 * MLP_H, "output-layer gradient layout");
+    const int rpb = rows_per_block(n), nb = (n + rpb - 1) / rpb;
+    hipLaunchKernelGGL(w4part_kernel, dim3(nb), dim3(256), 0, s, m->H3, m->dY, n, rpb, m->part);
+    hipLaunchKernelGGL(partsum_kernel, dim3((W4PART + 3) / 4), dim3(256), 0, s, (const float *)m->part, nb,
+                       W4PART, m->g + MLP_W4);
   }
-  gemm<false, true>(s, n, MLP_H, MLP_H, m->dZ3b, MLP_H, m->wb + MLP_W3, MLP_H, m->dH, MLP_H);
-  hipLaunchKernelGGL(silu_back_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->dH,
-                     m->Z2, n, m->dZf, m->dZ2b);
+  {
+    const int rpb = rows_per_block(n), nb = (n + rpb - 1) / rpb;
+    hipLaunchKernelGGL(out_back_kernel, dim3(nb), dim3(256), 0, s, m->dY, m->wb + MLP_W4, m->Z3, n, rpb,
+                       m->dZ3b, m->part);
+    if (wgrad) colsum(s, m->part, nb, m->g + MLP_B3);
+  }
+  const int nbt = (n + GT - 1) / GT;  // row tiles of the backward products (column partials)
+  if (wgrad)
+    gemm<true, false>(s, MLP_H, MLP_H, n, m->H2, MLP_H, m->dZ3b, MLP_H, m->g + MLP_W3, MLP_H, sk, m->part);
+  gemm<false, true, EPI_SILU_BACK>(s, n, MLP_H, MLP_H, m->dZ3b, MLP_H, m->wb + MLP_W3, MLP_H, m->part, MLP_H,
+                                   1, nullptr, nullptr, m->Z2, m->dZ2b);
   if (wgrad) {
-    hipLaunchKernelGGL(colsum_kernel, dim3(MLP_H), dim3(256), 0, s, m->dZf, n, MLP_H, m->g + MLP_B2);
+    colsum(s, m->part, nbt, m->g + MLP_B2);
     gemm<true, false>(s, MLP_H, MLP_H, n, m->H1, MLP_H, m->dZ2b, MLP_H, m->g + MLP_W2, MLP_H, sk, m->part);
   }
-  gemm<false, true>(s, n, MLP_H, MLP_H, m->dZ2b, MLP_H, m->wb + MLP_W2, MLP_H, m->dH, MLP_H);
-  hipLaunchKernelGGL(silu_back_kernel, dim3(blocks_for((size_t)n * MLP_H)), dim3(256), 0, s, m->dH,
-                     m->Z1, n, m->dZf, m->dZ1b);
+  gemm<false, true, EPI_SILU_BACK>(s, n, MLP_H, MLP_H, m->dZ2b, MLP_H, m->wb + MLP_W2, MLP_H, m->part, MLP_H,
+                                   1, nullptr, nullptr, m->Z1, m->dZ1b);
   if (wgrad) {
-    hipLaunchKernelGGL(colsum_kernel, dim3(MLP_H), dim3(256), 0, s, m->dZf, n, MLP_H, m->g + MLP_B1);
+    colsum(s, m->part, nbt, m->g + MLP_B1);
     gemm<true, false>(s, MLP_IN, MLP_H, n, m->X0, MLP_IN, m->dZ1b, MLP_H, m->g + MLP_W1, MLP_H, sk, m->part);
   }
   if (xgrad)
@@ -508,6 +645,8 @@ Mlp *mlp_create(int device, const MlpSpec &spec, const float *params, int max_ba
   MLP_OK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
   m->spec = spec;
   m->ds = device_spec(spec);
+  if (max_batch <= 0 || max_batch > 524288)
+    throw Error(MCCS_ERR_INVALID, "network batch must hold 1..524288 points");
   m->maxb = max_batch;
   const size_t B = (size_t)max_batch;
   m->w = m->alloc<float>(MLP_NPARAM);
@@ -525,8 +664,6 @@ Mlp *mlp_create(int device, const MlpSpec &spec, const float *params, int max_ba
   m->Z1 = m->alloc<float>(B * MLP_H);
   m->Z2 = m->alloc<float>(B * MLP_H);
   m->Z3 = m->alloc<float>(B * MLP_H);
-  m->dH = m->alloc<float>(B * MLP_H);
-  m->dZf = m->alloc<float>(B * MLP_H);
   m->dX = m->alloc<float>(B * MLP_IN);
   m->Y = m->alloc<float>(B * MLP_OUT);
   m->dY = m->alloc<float>(B * MLP_OUT);
@@ -603,10 +740,10 @@ double mlp_loss_grad(Mlp *m, const double *xyz, const double *e, uint64_t n, flo
     MLP_OK(hipMemsetAsync(m->g, 0, sizeof(float) * MLP_NPARAM, m->stream));
     MLP_OK(hipStreamSynchronize(m->stream));
   } catch (...) {
-    hipFree(dE);
+    (void)hipFree(dE);
     throw;
   }
-  hipFree(dE);
+  (void)hipFree(dE);
   return loss;
 }
 
@@ -665,10 +802,10 @@ double mlp_train(Mlp *m, const float *d_xyz, const float *d_e, uint64_t n, const
       if (history) history->push_back(last);
     }
   } catch (...) {
-    hipFree(d_perm);
+    (void)hipFree(d_perm);
     throw;
   }
-  hipFree(d_perm);
+  (void)hipFree(d_perm);
   return last;
 }
 
@@ -681,7 +818,7 @@ double mlp_train_host(Mlp *m, const double *xyz, const double *e, uint64_t n, co
     for (int k = 0; k < 2; k++) t[2 * i + k] = (float)(e[2 * i + k] / m->spec.out_scale[k]);
   float *dp = nullptr, *dt = nullptr;
   if (hipMalloc(&dp, sizeof(float) * 3 * n) != hipSuccess || hipMalloc(&dt, sizeof(float) * 2 * n) != hipSuccess) {
-    hipFree(dp);
+    (void)hipFree(dp);
     throw Error(MCCS_ERR_NOMEM, "device allocation of training data failed");
   }
   double last = 0.0;
@@ -690,12 +827,12 @@ double mlp_train_host(Mlp *m, const double *xyz, const double *e, uint64_t n, co
     MLP_OK(hipMemcpy(dt, t.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
     last = mlp_train(m, dp, dt, n, o, history);
   } catch (...) {
-    hipFree(dp);
-    hipFree(dt);
+    (void)hipFree(dp);
+    (void)hipFree(dt);
     throw;
   }
-  hipFree(dp);
-  hipFree(dt);
+  (void)hipFree(dp);
+  (void)hipFree(dt);
   return last;
 }
 

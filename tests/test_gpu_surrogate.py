@@ -11,7 +11,7 @@ import time
 import numpy as np
 import pytest
 
-from common import LONG, M, O, UBQ_PQR, globule_cfg, gpu_available, oracle_for, prepared
+from common import LONG, M, O, UBQ_PQR, globule_cfg, gpu_available, oracle_for, pair_scales, prepared
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import mlp_oracle as MO  # noqa: E402
@@ -52,7 +52,9 @@ def test_mfma_forward_and_position_gradient_match_oracle():
 def test_mfma_loss_gradient_matches_oracle():
     print("case mlp loss grad", flush=True)
     p, freq, centre, s, osc, rng = _net(1)
-    xyz = rng.uniform(-30, 30, (4096, 3))
+    # 4,093 points: the batch-wide products end in a partial k tile whose
+    # last chunk is not a whole 16-byte vector
+    xyz = rng.uniform(-30, 30, (4093, 3))
     e = np.stack([np.sin(xyz[:, 0] / 7.0) * 2.0, np.cos(xyz[:, 1] / 5.0) * 0.5], 1)
     spec = M.mlp_spec(freq.astype(np.float32), centre, s, osc)
     loss, g = M.mlp_loss_grad(spec, p, xyz, e)
@@ -295,23 +297,25 @@ def test_range_split_ccs_within_half_percent_on_the_50k_ellipsoid():
     assert t_sur < t_plain
 
 
-def test_batched_range_split_equals_single_target_runs(workdir):
+@pytest.mark.parametrize("lanes", [64, 32, 16])
+def test_batched_range_split_equals_single_target_runs(workdir, lanes):
     """The batch path with far fields (one persistent launch per probe kind,
-    graph replay) gives the records of single-target split runs byte for
-    byte; members without a dipole field keep the exact kernel."""
-    print("case batch split", flush=True)
+    graph replay) gives the records of single-target split runs of the same
+    group size byte for byte; members without a dipole field keep the exact
+    kernel."""
+    print("case batch split, %d lanes" % lanes, flush=True)
     cfgs = [globule_cfg(workdir),
             dict(targetFileName=UBQ_PQR, GasBuffer="He", polarizability="yes", seed=7, **LONG),
             dict(targetFileName=UBQ_PQR, GasBuffer="N2", polarizability="yes", seed=11, **LONG)]
     P = [prepared(c) for c in cfgs]
     us = [p.samples(n) for p, n in zip(P, (800, 1500, 600))]
-    B = M.Batch(P, lanes=64, skin=1.0)
+    B = M.Batch(P, lanes=lanes, skin=1.0)
     reps = B.build_surrogate(shell=6.0, blend=6.0, grid=0.5, exact_table=True)
     assert all(r["grid_nodes"] > 0 for r in reps)
     B.use_surrogate(True)
     recs = B.run(us)
     for p, u, r in zip(P, us, recs):
-        E = M.Engine(p, lanes=64, use_lds=0)
+        E = M.Engine(p, lanes=lanes, use_lds=0)
         E.build_surrogate(shell=6.0, blend=6.0, grid=0.5, exact_table=True)
         E.use_surrogate(True)
         ref = E.run(u)
@@ -321,8 +325,63 @@ def test_batched_range_split_equals_single_target_runs(workdir):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     B.use_surrogate(False)
     exact = B.run(us)
-    ref = M.Engine(P[0], lanes=64, use_lds=0).run(us[0])
+    ref = M.Engine(P[0], lanes=lanes, use_lds=0).run(us[0])
     assert np.array_equal(exact[0].view(np.uint8), ref.view(np.uint8))
+
+
+def test_split_site_forces_agree_across_lane_counts(workdir):
+    """The far field's 64 B-spline nodes are shared out over the lanes of a
+    trajectory's group (one node per lane at 64 lanes, two at 32, four at
+    16): the split site forces and energies at 32 and 16 lanes per point
+    equal those at 64 to 1e-12 of the summed magnitudes (only the order of
+    the sums differs)."""
+    print("case split lanes", flush=True)
+    cfg = globule_cfg(workdir)
+    P = prepared(cfg)
+    rng = np.random.default_rng(5)
+    a = P.atoms()
+    k = rng.integers(0, P.info.natoms, 200)
+    pts = np.stack([a["x"][k], a["y"][k], a["z"][k]], 1) + rng.normal(0, 6.0, (200, 3))
+    res = {}
+    for L in (64, 32, 16):
+        E = M.Engine(P, lanes=L)
+        E.build_surrogate(shell=6.0, blend=6.0, grid=0.5, exact_table=True)
+        E.use_surrogate(True)
+        res[L] = {site: E.eval_sites(site, pts) for site in (M.SITE_N, M.SITE_DUMMY)}
+    for site, qi, terms in [(M.SITE_N, -0.4825, (True, True, False)), (M.SITE_DUMMY, 0.965, (False, True, True))]:
+        f64, u64 = res[64][site]
+        for L in (32, 16):
+            f, U = res[L][site]
+            for j in range(len(pts)):
+                sf, su = pair_scales(P, cfg, pts[j], qi, *terms, cut=True)
+                scale_f = sf + np.abs(f64[j])
+                scale_u = su + abs(u64[j])
+                assert np.all(np.abs(f[j] - f64[j]) <= 1e-12 * scale_f), (L, site, j, f[j], f64[j])
+                assert abs(U[j] - u64[j]) <= 1e-12 * scale_u, (L, site, j, U[j], u64[j])
+
+
+def _check_directional_difference(spec, params, P):
+    """Training matches the network's directional derivative by a central
+    difference over +-0.2 A.  On the trained network that difference agrees
+    with the network's analytic gradient (its backward pass) to far better
+    than the 10 % error of the network's gradient against the exact field,
+    so the difference does not limit the fit."""
+    rng = np.random.default_rng(7)
+    a = P.atoms()
+    k = rng.integers(0, P.info.natoms, 4000)
+    pts = np.stack([a["x"][k], a["y"][k], a["z"][k]], 1) + rng.normal(0, 12.0, (4000, 3))
+    d = rng.normal(size=(4000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    h = 0.2
+    y, g = M.mlp_eval(spec, params, pts)
+    yp, _ = M.mlp_eval(spec, params, pts + h * d, grad=False)
+    ym, _ = M.mlp_eval(spec, params, pts - h * d, grad=False)
+    fd = (yp - ym) / (2 * h)
+    an = np.einsum("pod,pd->po", g, d)
+    rel = np.sqrt(np.mean((fd - an) ** 2)) / np.sqrt(np.mean(an ** 2))
+    print("network: central difference (h = 0.2 A) vs analytic directional derivative, rms %.2e" % rel,
+          flush=True)
+    assert rel <= 0.01
 
 
 def test_what_the_far_field_and_the_network_gain():
@@ -343,7 +402,9 @@ def test_what_the_far_field_and_the_network_gain():
     res = {}
     for name, kw in [("near only", dict(table_source=2)), ("exact table", dict(exact_table=True)),
                      ("network", dict(n_train=1000000, epochs=150))]:
-        rep, _, _ = E.build_surrogate(shell=6.0, blend=6.0, grid=0.5, **kw)
+        rep, params, spec = E.build_surrogate(shell=6.0, blend=6.0, grid=0.5, **kw)
+        if name == "network":
+            _check_directional_difference(spec, params, P)
         E.use_surrogate(True)
         rs = E.run(u)
         E.use_surrogate(False)
