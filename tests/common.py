@@ -132,7 +132,10 @@ def pair_scales(P, cfg, r, qi, lj, coul, dip, cut=True):
 # the integration itself.  The oracle runs it twice more: at dt/2 (every
 # try's step halved), and at dt from an impact parameter moved by four units
 # in the last place (4 x 2^-52 relative).  It counts as non-chaotic when the
-# tries and the fate are the same in all three runs and, if it scattered,
+# tries and the fate are the same in all three runs, the 4-ulp run takes
+# exactly as many steps as the dt run (an orbit around a charged target can
+# keep its tries and fate by chance while its length changes by thousands of
+# steps: every try of it is chaotic), and, if it scattered,
 # |chi(dt) - chi(dt/2)| <= CHAOS_DCHI (the step's truncation error stays
 # small) and |chi - chi(b + 4 ulp)| <= CHAOS_ULP_DCHI (a perturbation of the
 # size of a reordered FP64 sum is not amplified).  On a non-chaotic
@@ -180,7 +183,8 @@ def oracle_refs(P, cfg, u, key=None):
 
 def non_chaotic(ref, half, pert):
     same = ((ref["tries"] == half["tries"]) & (ref["success"] == half["success"]) &
-            (ref["tries"] == pert["tries"]) & (ref["success"] == pert["success"]))
+            (ref["tries"] == pert["tries"]) & (ref["success"] == pert["success"]) &
+            (ref["steps"] == pert["steps"]))
     # near-zero angles (free flights through the field, acos of 1 - ulp) are
     # compared as 1 - cos chi, as in the criterion itself
     big = np.maximum(ref["chi"], pert["chi"]) > CHI_SMALL

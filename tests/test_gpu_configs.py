@@ -63,12 +63,13 @@ def test_large_target_site_forces_match_oracle(which):
         print(which, kernel, "worst", worst, flush=True)
 
 
-@pytest.mark.parametrize("which,n,min_big", [("ellipsoid50k", 240, 5), ("shell1m", 40, 3)])
+@pytest.mark.parametrize("which,n,min_big", [("ellipsoid50k", 240, 5), ("shell1m", 40, 2)])
 def test_large_target_trajectories_match_oracle(which, n, min_big):
     """Hit flags everywhere, and every non-chaotic trajectory's tries, fate
     and angle (the oracle at dt, dt/2 and from a 4-ulp impact parameter).
     The shell's +60 e makes most of its trajectories chaotic orbits, and each
-    costs the oracle seconds: 40 of them."""
+    costs the oracle seconds: 40 of them, of which 11 are non-chaotic and 2
+    scatter by more than 1e-6 rad."""
     print("case trajectories", which, n, flush=True)
     P = _target(which)
     u = P.samples(n)
@@ -97,12 +98,13 @@ def test_shell_ccs_independent_of_summation_order():
 
 
 def test_assembly_batch_against_engines_and_oracle():
-    """Sixteen of the 64 assemblies (every fourth, 500-200,000 atoms, He type 4
-    and N2 type 6) in one batch: records byte-identical to per-target runs of
-    the same launch shape, and the smallest He and N2 members against the
-    oracle under the strict trajectory criterion."""
+    """Sixteen of the 64 assemblies (numbers 0, 1, 8, 9, ..., 57: 500-200,000
+    atoms; even numbers carry He, force type 4, odd ones N2, force type 6) in
+    one batch: records byte-identical to per-target runs of the same launch
+    shape, and the smallest He and N2 members against the oracle under the
+    strict trajectory criterion."""
     print("case assembly batch", flush=True)
-    descs = synth.assemblies()[::4]
+    descs = [d for k, d in enumerate(synth.assemblies()) if k % 8 in (0, 1)]
     P, cfgs, us = [], [], []
     for d in descs:
         s, xyz, q = synth.globule(n=d["n"], axes=d["axes"], net_charge=d["net_charge"], seed=d["seed"])
