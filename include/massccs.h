@@ -145,6 +145,11 @@ typedef struct {
                               is replaced by a walk over the cell columns, which differs only
                               in summation order.  No counterpart in the reference (it has
                               no lists). */
+  int32_t order;           /* order in which the run starts its trajectories: 0 = likely-long
+                              first (small impact parameter and slow speed: fewer groups sit
+                              idle at the end of the run), 1 = index order.  Records are the
+                              same either way.  No counterpart in the reference (its OpenMP
+                              loop schedules in index order). */
 } mccs_engine_opts;
 
 mccs_status mccs_engine_create(const mccs_prepared *p, const mccs_engine_opts *opts,
@@ -274,6 +279,7 @@ typedef struct {
   int32_t count_work;      /* 1: fill the work counters of the records */
   int32_t use_graph;       /* 1: capture the launches in a hipGraph and replay it */
   double skin;             /* Verlet-list skin in A; 0: tuned (or 1 A) */
+  int32_t order;           /* as mccs_engine_opts.order */
 } mccs_batch_opts;
 
 /* The targets' data is copied to the device: the mccs_prepared handles may be

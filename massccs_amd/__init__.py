@@ -59,12 +59,12 @@ class PreparedInfo(C.Structure):
 class EngineOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("block", C.c_int32),
                 ("use_lds", C.c_int32), ("count_work", C.c_int32), ("spec_below", C.c_int32),
-                ("list_cap", C.c_int32)]
+                ("list_cap", C.c_int32), ("order", C.c_int32)]
 
 
 class BatchOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("lanes", C.c_int32), ("count_work", C.c_int32),
-                ("use_graph", C.c_int32), ("skin", C.c_double)]
+                ("use_graph", C.c_int32), ("skin", C.c_double), ("order", C.c_int32)]
 
 
 class MlpSpec(C.Structure):
@@ -304,8 +304,9 @@ class Engine:
     """Device engine on one GPU (target resident in HBM)."""
 
     def __init__(self, prepared, device=0, lanes=0, block=0, use_lds=-1, count_work=True,
-                 spec_below=-1, list_cap=0):
-        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below, list_cap)
+                 spec_below=-1, list_cap=0, order=0):
+        o = EngineOpts(device, lanes, block, use_lds, 1 if count_work else 0, spec_below, list_cap,
+                       order)
         h = C.c_void_p()
         _check(lib().mccs_engine_create(prepared.h, C.byref(o), C.byref(h)))
         self.h, self.prepared = h, prepared
@@ -383,10 +384,10 @@ class Batch:
     kind / force type, replayed from a hipGraph)."""
 
     def __init__(self, prepared_list, device=0, lanes=0, count_work=True, use_graph=True,
-                 skin=0.0):
+                 skin=0.0, order=0):
         self.targets = list(prepared_list)
         arr = (C.c_void_p * len(self.targets))(*[p.h.value for p in self.targets])
-        o = BatchOpts(device, lanes, 1 if count_work else 0, 1 if use_graph else 0, skin)
+        o = BatchOpts(device, lanes, 1 if count_work else 0, 1 if use_graph else 0, skin, order)
         h = C.c_void_p()
         _check(lib().mccs_batch_create(arr, len(self.targets), C.byref(o), C.byref(h)))
         self.h = h

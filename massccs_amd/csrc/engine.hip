@@ -41,26 +41,43 @@ __global__ __launch_bounds__(512) void eval_kernel(Tgt T, const double *__restri
   }
 }
 
-// Tally: one block per iteration, thread 0 sums its trajectories in index
-// order exactly as src/System.cpp:287-307.
-__global__ void tally_iter_kernel(const mccs_traj_record *__restrict__ rec, uint64_t n_traj,
-                                  double *__restrict__ omega, int64_t *__restrict__ counts) {
-  if (threadIdx.x != 0) return;
+// Tally: one wavefront per iteration sums its trajectories' dOmega in index
+// order exactly as src/System.cpp:287-307 (a chain of additions in the
+// order of the records, as the reference's sequential loop): the wavefront
+// loads 64 records at a time and every lane adds the 64 values in lane
+// order from scalar registers (readlane_d); the three counts come from
+// ballots.
+
+__global__ __launch_bounds__(64) void tally_iter_kernel(const mccs_traj_record *__restrict__ rec,
+                                                        uint64_t n_traj, double *__restrict__ omega,
+                                                        int64_t *__restrict__ counts) {
   const uint64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
   const mccs_traj_record *r = rec + i * n_traj;
   double dO = 0.0;
-  int nf = 0, ns = 0, nl = 0;
-  for (uint64_t k = 0; k < n_traj; k++) {
-    const mccs_traj_record x = r[k];
-    dO += x.domega;
-    if (!x.hit) nf++;
-    else if (x.success) ns++;
-    else nl++;
+  int64_t nf = 0, ns = 0, nl = 0;
+  for (uint64_t base = 0; base < n_traj; base += 64) {
+    const uint64_t k = base + lane;
+    const bool valid = k < n_traj;
+    double d = 0.0;
+    int hit = 0, suc = 0;
+    if (valid) {
+      d = r[k].domega;
+      hit = r[k].hit;
+      suc = r[k].success;
+    }
+    nf += __popcll(__ballot(valid && !hit));
+    ns += __popcll(__ballot(valid && hit && suc));
+    nl += __popcll(__ballot(valid && hit && !suc));
+    const int m = (int)(n_traj - base < 64 ? n_traj - base : 64);
+    for (int l = 0; l < m; l++) dO += readlane_d(d, l);
   }
-  omega[i] = 1.0 / ((double)(float)(ns + nf)) * dO;
-  counts[3 * i] = nf;
-  counts[3 * i + 1] = ns;
-  counts[3 * i + 2] = nl;
+  if (lane == 0) {
+    omega[i] = 1.0 / ((double)(float)(ns + nf)) * dO;
+    counts[3 * i] = nf;
+    counts[3 * i + 1] = ns;
+    counts[3 * i + 2] = nl;
+  }
 }
 
 // Mean and standard error over iterations, src/System.cpp:295-317.
@@ -145,18 +162,19 @@ constexpr int QCTL_WORDS = 6 * QLINE;
 constexpr uint64_t SLOT_SLACK = 8192;  // spawn slots beyond 8 per trajectory
 
 // Per trajectory: TRIES_MAX try outcomes, 8 spawn slots, claimed tries,
-// spawned and resolved flags.
+// spawned and resolved flags, its place in the start order.
 constexpr size_t QUEUE_PER_TRAJ =
-    TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int);
-// Workspace bytes of a queue over n trajectories.
+    TRIES_MAX * sizeof(TryOut) + 8 * sizeof(unsigned long long) + 3 * sizeof(int) + sizeof(unsigned);
+// Workspace bytes of a queue over n trajectories (with the start order's
+// bucket counters).
 constexpr size_t queue_ws_bytes(uint64_t n) {
-  return QUEUE_PER_TRAJ * n + sizeof(unsigned long long) * SLOT_SLACK;
+  return QUEUE_PER_TRAJ * n + sizeof(unsigned long long) * SLOT_SLACK + sizeof(unsigned) * ORDER_BUCKETS;
 }
 
 // Queue of n trajectories over a counter block ctl (QCTL_WORDS words) and a
 // workspace ws of queue_ws_bytes(n) bytes, both device memory.
 static Queue queue_over(unsigned long long *ctl, char *ws, uint64_t n, long long spec_below,
-                        unsigned long long wait_ticks) {
+                        unsigned long long wait_ticks, bool ordered) {
   Queue Q;
   Q.ticket = ctl + 0 * QLINE;
   Q.shead = ctl + 1 * QLINE;
@@ -170,6 +188,9 @@ static Queue queue_over(unsigned long long *ctl, char *ws, uint64_t n, long long
   Q.tries = reinterpret_cast<int *>(Q.slots + Q.nslots);
   Q.spawned = Q.tries + n;
   Q.resolved = Q.spawned + n;
+  unsigned *ord = reinterpret_cast<unsigned *>(Q.resolved + n);
+  Q.order = ordered ? ord : nullptr;
+  Q.bucket = ordered ? ord + n : nullptr;
   Q.wait_ticks = wait_ticks;
   Q.spec_below = spec_below;
   return Q;
@@ -217,8 +238,9 @@ struct QueueWS {
     HIP_OK(hipMemset(qctl, 0, sizeof(unsigned long long) * QCTL_WORDS));
     wait_ticks = queue_wait_ticks(dev);
   }
-  // Queue over n trajectories; spawning below spec_below unresolved ones.
-  Queue make(uint64_t n, long long spec_below) {
+  // Queue over n trajectories; spawning below spec_below unresolved ones;
+  // primary trajectories taken in the start order when ordered.
+  Queue make(uint64_t n, long long spec_below, bool ordered) {
     if (n > qcap) {
       if (qws) HIP_OK(hipFree(qws));
       qws = nullptr;
@@ -227,7 +249,7 @@ struct QueueWS {
       HIP_OK(hipMalloc(&qws, queue_ws_bytes(cap)));
       qcap = cap;
     }
-    return queue_over(qctl, qws, n, spec_below, wait_ticks);
+    return queue_over(qctl, qws, n, spec_below, wait_ticks, ordered);
   }
   // Throws when a group of the last run gave up waiting for work (its
   // records are then incomplete).  The run must have completed.
@@ -332,6 +354,7 @@ struct Engine {
   bool sur_built = false;
   bool split = false;       // runs use Ts
   int spec_below = -1;
+  bool ordered = true;      // primary trajectories in the start order (likely-long first)
   hipStream_t last_stream = nullptr;
   const unsigned long long *last_ctl = nullptr;  // queue counters of the last run
   double list_need = 0.0;  // estimated largest Verlet list at skin 1 A (L2-fed targets)
@@ -379,6 +402,8 @@ Engine *engine_create(const Prepared &P, const mccs_engine_opts &o) {
   if (E->gk == 1 && E->ft > 4) E->ft = 2;
   E->count = o.count_work != 0;
   E->spec_below = o.spec_below;
+  if (o.order < 0 || o.order > 1) throw Error(MCCS_ERR_INVALID, "order must be 0 or 1");
+  E->ordered = o.order == 0;
 
   // per-atom LJ pairs in the reference's order (src/Force.cpp:84-85):
   // lj1 = 4 eps sig^6, lj2 = lj1 sig^6; the second pair for the CO2 carbon site
@@ -507,6 +532,20 @@ static void launch_queue_init(Queue Q, uint64_t n, hipStream_t s) {
   HIP_OK(hipLaunchKernel((const void *)queue_init, dim3(256), dim3(256), args, 0, s));
 }
 
+// The start order of a queue that has one (after launch_queue_init, which
+// clears its bucket counters): count, scan, scatter.
+static void launch_order(Queue Q, const Tgt *tg, const unsigned long long *toff, int ntg,
+                         const double *u, uint64_t n, hipStream_t s) {
+  if (!Q.order || n == 0) return;
+  if (n > 0xffffffffull) throw Error(MCCS_ERR_INVALID, "too many trajectories for one launch");
+  const dim3 grid((unsigned)((n + 255) / 256));
+  void *args[] = {&tg, &toff, &ntg, &u, &n, &Q};
+  HIP_OK(hipLaunchKernel((const void *)order_count, grid, dim3(256), args, 0, s));
+  void *sargs[] = {&Q};
+  HIP_OK(hipLaunchKernel((const void *)order_scan, dim3(1), dim3(ORDER_BUCKETS), sargs, 0, s));
+  HIP_OK(hipLaunchKernel((const void *)order_scatter, grid, dim3(256), args, 0, s));
+}
+
 static void set_shmem(const void *fn, size_t bytes) {
   if (bytes > 64 * 1024)
     HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -549,6 +588,7 @@ static void launch_traj(Engine *E, const TrajLaunch &tl, Queue Q, const double *
   const Tgt *tg = E->d_tg;
   const unsigned long long *toff = nullptr;
   int ntg = 1;
+  launch_order(Q, tg, toff, ntg, u_dev, n, s);
   void *args[] = {&tg, &toff, &ntg, &u_dev, &n, &rec_dev, &Q};
   HIP_OK(hipLaunchKernel((const void *)tl.fn, dim3((unsigned)tl.blocks), dim3(E->block), args,
                          tl.shmem, s));
@@ -561,7 +601,7 @@ void engine_run_device(Engine *E, const double *u_dev, uint64_t n, mccs_traj_rec
   HIP_OK(hipSetDevice(E->device));
   hipStream_t s = stream ? (hipStream_t)stream : E->stream;
   const TrajLaunch tl = traj_launch(E, n);
-  Queue Q = E->q.make(n, tl.spec_below);
+  Queue Q = E->q.make(n, tl.spec_below, E->ordered);
   launch_traj(E, tl, Q, u_dev, n, rec_dev, s);
   E->last_ctl = E->q.qctl;
 }
@@ -623,7 +663,7 @@ void engine_run_device_noalloc(Engine *E, const double *u_dev, uint64_t n,
   // queue_init (a kernel) clears the counters and every per-trajectory entry
   // the run reads before the trajectory kernel starts
   Queue Q = queue_over(ctl, reinterpret_cast<char *>(d.ptr + NOALLOC_CTL_BYTES), n, tl.spec_below,
-                       d.wait_ticks);
+                       d.wait_ticks, E->ordered);
   launch_traj(E, tl, Q, u_dev, n, rec_dev, s);
   E->last_ctl = ctl;
 }
@@ -1334,6 +1374,7 @@ struct Batch {
   int device = 0, cus = 0;
   bool count = true, use_graph = true;
   bool split = false;  // far-field surrogates in use (where built)
+  bool ordered = true;  // primary trajectories in the start order (likely-long first)
   int lanes_opt = 0;
   double skin_opt = 0.0;
   hipStream_t stream = nullptr;
@@ -1428,6 +1469,8 @@ Batch *batch_create(const Prepared *const *targets, int ntargets, const mccs_bat
   B->use_graph = o.use_graph != 0;
   B->lanes_opt = o.lanes;
   B->skin_opt = o.skin;
+  if (o.order < 0 || o.order > 1) throw Error(MCCS_ERR_INVALID, "order must be 0 or 1");
+  B->ordered = o.order == 0;
   if (o.lanes != 0 && o.lanes != 16 && o.lanes != 32 && o.lanes != 64)
     throw Error(MCCS_ERR_INVALID, "lanes must be 0, 16, 32 or 64");
   if (o.skin < 0.0) throw Error(MCCS_ERR_INVALID, "skin must be >= 0 (0 = tuned)");
@@ -1512,13 +1555,14 @@ static void launch_class(Batch &B, BatchClass &c, hipStream_t s) {
   if (c.n == 0) return;
   const uint64_t gpb = c.block / c.lanes;
   const uint64_t blocks = std::min<uint64_t>(c.blocks, (c.n + gpb - 1) / gpb);
-  Queue Q = c.q.make(c.n, (long long)(blocks * gpb));
+  Queue Q = c.q.make(c.n, (long long)(blocks * gpb), B.ordered);
   launch_queue_init(Q, c.n, s);
   const Tgt *tg = c.d_tg;
   const unsigned long long *toff = c.d_toff;
   int ntg = (int)c.members.size();
   const double *u = c.d_u;
   uint64_t n = c.n;
+  launch_order(Q, tg, toff, ntg, u, n, s);
   mccs_traj_record *rec = c.d_rec;
   void *args[] = {&tg, &toff, &ntg, &u, &n, &rec, &Q};
   HIP_OK(hipLaunchKernel((const void *)c.fn, dim3((unsigned)blocks), dim3(c.block), args, c.shmem, s));
@@ -1544,7 +1588,7 @@ static void run_classes(Batch &B, bool sizes_changed) {
     B.gexec = nullptr;
     B.graph = nullptr;
     // every queue workspace must exist before capture (no allocation inside)
-    for (auto &c : B.cls) c.q.make(c.n, 0);
+    for (auto &c : B.cls) c.q.make(c.n, 0, B.ordered);
     HIP_OK(hipStreamSynchronize(B.stream));
     HIP_OK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
     try {

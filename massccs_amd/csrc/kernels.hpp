@@ -1366,9 +1366,17 @@ struct Queue {
   int *spawned;                // 1 once the trajectory's remaining tries are spawned
   int *resolved;               // 1 once the trajectory's record is written
   TryOut *tab;                 // TRIES_MAX per trajectory
+  // Primary ticket t takes trajectory order[t] (null: trajectory t).  The
+  // order only decides when a trajectory starts: every try is a pure
+  // function of (trajectory, try), so the records are the same in any order.
+  const unsigned *order;
+  unsigned *bucket;            // ORDER_BUCKETS counters of the ordering pass
   unsigned long long wait_ticks;  // wall-clock ticks a group waits for work at most
   long long spec_below;        // spawning starts below this many unresolved trajectories
 };
+
+// Buckets of the start order (order_count / order_scan / order_scatter).
+constexpr int ORDER_BUCKETS = 1024;
 
 // A first try that has run this many steps is likely to orbit and restart.
 constexpr int SPAWN_STEPS = 16384;
@@ -1588,7 +1596,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
         }
         t = bcast<L>(t);
         if (t < n) {
-          j = t;
+          j = Q.order ? Q.order[t] : t;
           k = 0;
           tj = target_of(j);
           w = Work{0, 0, 0, 0};
@@ -1992,6 +2000,63 @@ static __global__ void queue_init(Queue Q, uint64_t n) {
     for (int k = 0; k < 5; k++) Q.work[k] = 0;
     *Q.stalled = 0;
   }
+  if (Q.bucket)
+    for (uint64_t s = i; s < (uint64_t)ORDER_BUCKETS; s += stride) Q.bucket[s] = 0;
+}
+
+// ---------------------------------------------------------------- start order
+// Long trajectories started last make the launch's tail: the other groups
+// sit idle while a few long tries finish.  Trajectories with a small impact
+// parameter and a slow speed are the likely long ones (they reach the
+// target's core and orbit), so primary tickets take them first: a counting
+// sort of the trajectories by key = (b / bmax)^2 u_v (u_v the speed draw,
+// whose speed grows with it) into ORDER_BUCKETS buckets, ascending.  The
+// order inside a bucket is that of the atomics: any order gives the same
+// records.
+
+__device__ __forceinline__ int order_bucket(const Tgt *tg, const unsigned long long *toff, int ntg,
+                                            const double *u, uint64_t j) {
+  int lo = 0, hi = ntg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (toff[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  const double x = u[9 * j] / tg[lo].bmax, v = u[9 * j + 1];
+  const double key = x * x * v;  // in [0, 1] for b in [0, bmax]; NaN-safe below
+  const int b = (int)(key * ORDER_BUCKETS);
+  return b < 0 ? 0 : (b >= ORDER_BUCKETS ? ORDER_BUCKETS - 1 : b);
+}
+
+static __global__ void order_count(const Tgt *tg, const unsigned long long *toff, int ntg,
+                                   const double *u, uint64_t n, Queue Q) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) atomicAdd(Q.bucket + order_bucket(tg, toff, ntg, u, j), 1u);
+}
+
+// Exclusive prefix sum of the bucket counts in place (one block of
+// ORDER_BUCKETS threads).
+static __global__ __launch_bounds__(ORDER_BUCKETS) void order_scan(Queue Q) {
+  __shared__ unsigned sc[ORDER_BUCKETS];
+  const int i = threadIdx.x;
+  const unsigned c = Q.bucket[i];
+  sc[i] = c;
+  __syncthreads();
+  for (int off = 1; off < ORDER_BUCKETS; off <<= 1) {
+    const unsigned add = i >= off ? sc[i - off] : 0u;
+    __syncthreads();
+    sc[i] += add;
+    __syncthreads();
+  }
+  Q.bucket[i] = sc[i] - c;
+}
+
+static __global__ void order_scatter(const Tgt *tg, const unsigned long long *toff, int ntg,
+                                     const double *u, uint64_t n, Queue Q) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const unsigned pos = atomicAdd(Q.bucket + order_bucket(tg, toff, ntg, u, j), 1u);
+  const_cast<unsigned *>(Q.order)[pos] = (unsigned)j;
 }
 
 using TrajFn = void (*)(const Tgt *, const unsigned long long *, int, const double *, uint64_t,

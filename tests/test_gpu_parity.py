@@ -250,14 +250,18 @@ def test_restarts_run_side_by_side_with_identical_results(workdir):
     """Later tries of a restarting trajectory may run concurrently on idle
     groups; the fold over their outcomes must give exactly the records of
     running them one after another, whatever the schedule (each try builds
-    its own neighbour list, so even the rounding is schedule-independent)."""
+    its own neighbour list, so even the rounding is schedule-independent).
+    Likewise the order in which trajectories start (likely-long first, or
+    index order) leaves every record unchanged."""
     cfg = globule_cfg(workdir)
     print("case globule_N2 restarts", flush=True)
     P = prepared(cfg)
     u = P.samples(4000)
     runs = []
-    for spec in (0, -1, 10 ** 9):
-        E = M.Engine(P, spec_below=spec)
+    # the three spawning thresholds in the start order (likely-long first),
+    # then once in index order
+    for spec, order in ((0, 0), (-1, 0), (10 ** 9, 0), (-1, 1)):
+        E = M.Engine(P, spec_below=spec, order=order)
         runs.append(E.run(u))
         w = E.last_work()
         assert w["steps"] >= runs[-1]["steps"].sum()
