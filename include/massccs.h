@@ -345,6 +345,18 @@ mccs_status mccs_mlp_loss_grad(int32_t device, const mccs_mlp_spec *spec, const 
                                const double *xyz, const double *e, uint64_t n, double *loss,
                                float *grad);
 
+/* Loss of values and directional derivatives over one batch (n <= 32768):
+ * sum over points and both outputs of (y - e)^2 + w (dy - d)^2 / 2n in the
+ * scaled output units, y the outputs at xyz and dy their derivatives along
+ * the unit vectors dir (n x 3), carried through the network in forward mode
+ * (the analytic directional derivative); e (n x 2, kcal/mol) and d (n x 2,
+ * kcal/mol/A) are the targets.  grad receives the loss gradient with respect
+ * to the parameters; y_out and dy_out (may be NULL) the scaled y and dy. */
+mccs_status mccs_mlp_jvp_loss_grad(int32_t device, const mccs_mlp_spec *spec, const float *params,
+                                   const double *xyz, const double *dir, const double *e,
+                                   const double *d, double w, uint64_t n, double *loss,
+                                   float *grad, double *y_out, double *dy_out);
+
 /* Adam training (cosine-decayed learning rate lr -> lr/20) of params in place
  * on n points; loss_history (may be NULL) receives one mean loss per epoch. */
 mccs_status mccs_mlp_train(int32_t device, const mccs_mlp_spec *spec, float *params,

@@ -103,7 +103,8 @@ EXPORTS = [
     "mccs_engine_destroy", "mccs_run", "mccs_run_device", "mccs_eval_sites", "mccs_tally",
     "mccs_tally_device", "mccs_version", "mccs_engine_last_work", "mccs_batch_create",
     "mccs_batch_destroy", "mccs_batch_run", "mccs_batch_autotune", "mccs_batch_shapes",
-    "mccs_batch_last_work", "mccs_mlp_eval", "mccs_mlp_loss_grad", "mccs_mlp_train",
+    "mccs_batch_last_work", "mccs_mlp_eval", "mccs_mlp_loss_grad", "mccs_mlp_jvp_loss_grad",
+    "mccs_mlp_train",
     "mccs_engine_build_surrogate", "mccs_engine_use_surrogate", "mccs_run_device_noalloc",
     "mccs_batch_build_surrogate", "mccs_batch_use_surrogate", "mccs_batch_run_device",
     "mccs_cells_device", "mccs_inertia_device",
@@ -162,6 +163,9 @@ def lib():
                                 C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mccs_mlp_loss_grad.argtypes = [C.c_int32, C.POINTER(MlpSpec), fp, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double), fp]
+    dp = C.POINTER(C.c_double)
+    L.mccs_mlp_jvp_loss_grad.argtypes = [C.c_int32, C.POINTER(MlpSpec), fp, dp, dp, dp, dp, C.c_double,
+                                         C.c_uint64, dp, fp, dp, dp]
     L.mccs_mlp_train.argtypes = [C.c_int32, C.POINTER(MlpSpec), fp, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double), C.c_uint64, C.c_int32, C.c_int32,
                                  C.c_double, C.c_uint32, C.POINTER(C.c_double)]
@@ -499,6 +503,24 @@ def mlp_loss_grad(spec, params, xyz, e, device=0):
     _check(lib().mccs_mlp_loss_grad(device, C.byref(spec), _fp(params), _dp(xyz), _dp(e),
                                     len(xyz), C.byref(loss), _fp(g)))
     return loss.value, g
+
+
+def mlp_jvp_loss_grad(spec, params, xyz, dirs, e, d, w, device=0):
+    """Loss of values and forward-mode directional derivatives and its
+    parameter gradient; also the scaled outputs y and derivatives dy."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    e = np.ascontiguousarray(e, dtype=np.float64).reshape(-1, 2)
+    d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 2)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    y = np.zeros((len(xyz), 2))
+    dy = np.zeros((len(xyz), 2))
+    loss = C.c_double()
+    _check(lib().mccs_mlp_jvp_loss_grad(device, C.byref(spec), _fp(params), _dp(xyz), _dp(dirs),
+                                        _dp(e), _dp(d), float(w), len(xyz), C.byref(loss), _fp(g),
+                                        _dp(y), _dp(dy)))
+    return loss.value, g, y, dy
 
 
 def mlp_train(spec, params, xyz, e, epochs=10, batch=4096, lr=2e-3, seed=1, device=0):

@@ -313,6 +313,18 @@ mccs_status mccs_mlp_loss_grad(int32_t device, const mccs_mlp_spec *spec, const 
   });
 }
 
+mccs_status mccs_mlp_jvp_loss_grad(int32_t device, const mccs_mlp_spec *spec, const float *params,
+                                   const double *xyz, const double *dir, const double *e,
+                                   const double *d, double w, uint64_t n, double *loss,
+                                   float *grad, double *y_out, double *dy_out) {
+  return guard([&] {
+    need(spec && params && xyz && dir && e && d && loss && grad, "null argument");
+    need(n > 0 && n <= 32768, "batch must hold 1..32768 points");
+    MlpHolder h{mccs::mlp_create(device, to_spec(*spec), params, (int)(2 * n))};
+    *loss = mccs::mlp_jvp_loss_grad(h.m, xyz, dir, e, d, w, n, grad, y_out, dy_out);
+  });
+}
+
 mccs_status mccs_mlp_train(int32_t device, const mccs_mlp_spec *spec, float *params,
                            const double *xyz, const double *e, uint64_t n, int32_t epochs,
                            int32_t batch, double lr, uint32_t seed, double *loss_history) {

@@ -1167,8 +1167,8 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
     E->mlp = nullptr;
     E->mlp = mlp_create(E->device, spec, params.data(), 65536);
     // training data: positions, scaled psi, and the directional derivative of
-    // psi along a random unit vector per point (matched by central
-    // differences of the network)
+    // psi along a random unit vector per point (matched by the network's
+    // forward-mode derivative along it)
     DevBuf<float> txyz(3 * ntr), te(2 * ntr), tdir(3 * ntr), tder(2 * ntr);
     double wd = 1.0;
     {
@@ -1210,7 +1210,6 @@ void engine_build_surrogate(Engine *E, const mccs_surrogate_opts &o, mccs_surrog
     to.seed = o.seed ? o.seed : 2104;
     to.d_dir = tdir;
     to.d_deriv = tder;
-    to.fd_h = 0.2;
     to.w_deriv = wd;
     rep.train_loss = mlp_train(E->mlp, txyz, te, ntr, to, nullptr);
     rep.seconds_train = secs_since(t1);

@@ -50,6 +50,11 @@ struct DSpec {
 };
 
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// SiLU'(z) and SiLU''(z) from sg = sigm(z)
+__device__ __forceinline__ float silu1(float z, float sg) { return sg * (1.f + z * (1.f - sg)); }
+__device__ __forceinline__ float silu2(float z, float sg) {
+  return sg * (1.f - sg) * (2.f + z * (1.f - 2.f * sg));
+}
 
 // 64 features per point: column f = sin(2 pi B_f . r'), column 32 + f = cos.
 __global__ void features_kernel(const float *__restrict__ xyz, int n, DSpec sp,
@@ -89,8 +94,14 @@ __global__ void features_kernel(const float *__restrict__ xyz, int n, DSpec sp,
 // bias[n] and H = bf16(SiLU(Z)); EPI_SILU_BACK (backward layer) H = bf16(dZ),
 // dZ = product * SiLU'(Zin), and C receives the block's column sums of dZ
 // over its GT rows: C[blockIdx.y][n] (the partials of the bias gradient).
+// Tangent rows (forward-mode derivative): EPI_TAN C = product and H =
+// bf16(product * SiLU'(Zin)), Zin the value rows' pre-activations (forward:
+// dZ -> dH; backward: the tangent's upstream gradient -> its dZ);
+// EPI_SILU2_BACK as EPI_SILU_BACK with dZ = product * SiLU'(Zin) + Gt *
+// SiLU''(Zin) * Zt (Zt the tangent pre-activations, Gt the tangent's
+// upstream gradient), for the value rows of a tangent-carrying backward.
 constexpr int GT = 64, GK = 64, GP = GK + 8;
-enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_SILU_BACK = 2 };
+enum { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_SILU_BACK = 2, EPI_TAN = 3, EPI_SILU2_BACK = 4 };
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
@@ -147,7 +158,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
                                                    float *__restrict__ C, int ldc, int kchunk,
                                                    const float *__restrict__ bias,
                                                    const float *__restrict__ Zin,
-                                                   uint16_t *__restrict__ H) {
+                                                   uint16_t *__restrict__ H,
+                                                   const float *__restrict__ Zt,
+                                                   const float *__restrict__ Gt) {
   __shared__ __align__(16) uint16_t As[GT * GP];
   __shared__ __align__(16) uint16_t Bs[GT * GP];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -190,7 +203,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
     }
   }
-  if (EPI == EPI_SILU_BACK) {
+  if (EPI == EPI_SILU_BACK || EPI == EPI_SILU2_BACK) {
     // column sums over the wavefront's 32 rows: the two tiles of a lane,
     // then the 16 lanes of equal q (rows l % 16), read from lane q * 16
     float cs[2][4];
@@ -205,8 +218,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
           const int nn = bn + wn + 16 * j + 4 * q + e;
           float dz = 0.f;
           if (m < M && nn < N) {
-            const float z = Zin[(size_t)m * ldc + nn], sg = sigm(z);
-            dz = acc[i][j][e] * sg * (1.f + z * (1.f - sg));
+            const size_t o = (size_t)m * ldc + nn;
+            const float z = Zin[o], sg = sigm(z);
+            dz = acc[i][j][e] * silu1(z, sg);
+            if (EPI == EPI_SILU2_BACK) dz += Gt[o] * silu2(z, sg) * Zt[o];
           }
           acc[i][j][e] = dz;
           v += dz;
@@ -242,13 +257,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
         const f4 bb = *reinterpret_cast<const f4 *>(bias + n0);
         v += bb;
         *reinterpret_cast<f4 *>(C + o) = v;
-      } else if (EPI == EPI_STORE) {
+      } else if (EPI == EPI_STORE || EPI == EPI_TAN) {
         *reinterpret_cast<f4 *>(C + o) = v;
       }
       if (EPI != EPI_STORE) {
         uint16_t hv[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) hv[e] = to_bf16(EPI == EPI_BIAS_SILU ? v[e] * sigm(v[e]) : v[e]);
+        for (int e = 0; e < 4; e++) {
+          float h = v[e];
+          if (EPI == EPI_BIAS_SILU) h = v[e] * sigm(v[e]);
+          if (EPI == EPI_TAN) {
+            const float z = Zin[o + e];
+            h = v[e] * silu1(z, sigm(z));
+          }
+          hv[e] = to_bf16(h);
+        }
         uint2 packed;
         packed.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
         packed.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
@@ -282,7 +305,7 @@ __global__ __launch_bounds__(256) void partsum_kernel(const float *__restrict__ 
 // sum_p dY[p][o] (the bias).
 constexpr int W4PART = 2 * MLP_H + 2;
 __global__ __launch_bounds__(256) void w4part_kernel(const uint16_t *__restrict__ H, const float *__restrict__ dY,
-                                                     int n, int rpb, float *__restrict__ part) {
+                                                     int n, int nb, int rpb, float *__restrict__ part) {
   __shared__ float red[256][3];
   const int j = threadIdx.x % MLP_H, h = threadIdx.x / MLP_H;
   const int r0 = blockIdx.x * rpb, r1 = min(n, r0 + rpb);
@@ -292,8 +315,10 @@ __global__ __launch_bounds__(256) void w4part_kernel(const uint16_t *__restrict_
     const float d0 = dY[2 * r], d1 = dY[2 * r + 1];
     s0 += hv * d0;
     s1 += hv * d1;
-    sb0 += d0;
-    sb1 += d1;
+    if (r < nb) {  // tangent rows carry no bias
+      sb0 += d0;
+      sb1 += d1;
+    }
   }
   red[threadIdx.x][0] = s0;
   red[threadIdx.x][1] = s1;
@@ -310,9 +335,10 @@ __global__ __launch_bounds__(256) void w4part_kernel(const uint16_t *__restrict_
   }
 }
 
-// Y = H3 W4 + b4 (BF16 operands, FP32 sums).
+// Y = H3 W4 + b4 (BF16 operands, FP32 sums); rows from nb on are tangent
+// rows, which carry no bias.
 __global__ void out_layer_kernel(const uint16_t *__restrict__ H, const uint16_t *__restrict__ W4,
-                                 const float *__restrict__ b4, int n, float *__restrict__ Y) {
+                                 const float *__restrict__ b4, int n, int nb, float *__restrict__ Y) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   float y0 = 0.f, y1 = 0.f;
@@ -321,8 +347,8 @@ __global__ void out_layer_kernel(const uint16_t *__restrict__ H, const uint16_t 
     y0 += h * from_bf16(W4[2 * j]);
     y1 += h * from_bf16(W4[2 * j + 1]);
   }
-  Y[2 * p] = y0 + b4[0];
-  Y[2 * p + 1] = y1 + b4[1];
+  Y[2 * p] = p < nb ? y0 + b4[0] : y0;
+  Y[2 * p + 1] = p < nb ? y1 + b4[1] : y1;
 }
 
 // dY = d(mean squared error)/dY; loss accumulated in double.
@@ -437,38 +463,46 @@ __global__ void gather_kernel(const float *__restrict__ xyz, const int *__restri
   out[3 * p + 2] = xyz[3 * g + 2];
 }
 
-// Rows [0, B): the points; [B, 2B): point + h dir; [2B, 3B): point - h dir.
-__global__ void sobolev_pts_kernel(const float *__restrict__ xyz, const float *__restrict__ dir,
-                                   const int *__restrict__ idx, int B, float h, float *__restrict__ pts) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Value rows [0, B) and tangent rows [B, 2B) of the features of points
+// xyz[idx[p]] (idx may be null) along unit directions dir[idx[p]]: sin a,
+// cos a and their derivatives cos a da, -sin a da with da = 2 pi s B . u.
+__global__ void jvp_features_kernel(const float *__restrict__ xyz, const float *__restrict__ dir,
+                                    const int *__restrict__ idx, int B, DSpec sp,
+                                    uint16_t *__restrict__ X0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = i / MLP_NF, f = i % MLP_NF;
   if (p >= B) return;
-  const int g = idx[p];
-  for (int d = 0; d < 3; d++) {
-    const float r = xyz[3 * g + d], u = dir[3 * g + d];
-    pts[3 * p + d] = r;
-    pts[3 * (B + p) + d] = r + h * u;
-    pts[3 * (2 * B + p) + d] = r - h * u;
-  }
+  const int g = idx ? idx[p] : p;
+  const float x = (xyz[3 * g] - sp.c[0]) * sp.s;
+  const float y = (xyz[3 * g + 1] - sp.c[1]) * sp.s;
+  const float z = (xyz[3 * g + 2] - sp.c[2]) * sp.s;
+  const float a = TWO_PI * (sp.freq[f][0] * x + sp.freq[f][1] * y + sp.freq[f][2] * z);
+  const float da = TWO_PI * sp.s *
+                   (sp.freq[f][0] * dir[3 * g] + sp.freq[f][1] * dir[3 * g + 1] + sp.freq[f][2] * dir[3 * g + 2]);
+  float sv, cv;
+  sincosf(a, &sv, &cv);
+  X0[(size_t)p * MLP_IN + f] = to_bf16(sv);
+  X0[(size_t)p * MLP_IN + MLP_NF + f] = to_bf16(cv);
+  X0[(size_t)(B + p) * MLP_IN + f] = to_bf16(cv * da);
+  X0[(size_t)(B + p) * MLP_IN + MLP_NF + f] = to_bf16(-sv * da);
 }
 
-// Energy error at the points plus w times the error of the central-difference
-// directional derivative; seeds dY of all 3B rows.
-__global__ void sobolev_loss_kernel(const float *__restrict__ Y, const float *__restrict__ E,
-                                    const float *__restrict__ T, const int *__restrict__ idx, int B,
-                                    float inv2h, float inv, float w, float *__restrict__ dY,
-                                    double *__restrict__ loss) {
+// Loss of value rows Y[0, B) against E and of tangent rows Y[B, 2B) against
+// the directional derivatives T (weight w), per point g = idx[p]; seeds dY
+// of all 2B rows.
+__global__ void jvp_loss_kernel(const float *__restrict__ Y, const float *__restrict__ E,
+                                const float *__restrict__ T, const int *__restrict__ idx, int B,
+                                float inv, float w, float *__restrict__ dY, double *__restrict__ loss) {
   __shared__ float red[256];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (p < B) {
-    const int g = idx[p];
+    const int g = idx ? idx[p] : p;
     for (int o = 0; o < 2; o++) {
       const float re = Y[2 * p + o] - E[2 * g + o];
-      const float D = (Y[2 * (B + p) + o] - Y[2 * (2 * B + p) + o]) * inv2h;
-      const float rd = D - T[2 * g + o];
+      const float rd = Y[2 * (B + p) + o] - T[2 * g + o];
       dY[2 * p + o] = 2.f * re * inv;
-      dY[2 * (B + p) + o] = 2.f * w * rd * inv2h * inv;
-      dY[2 * (2 * B + p) + o] = -2.f * w * rd * inv2h * inv;
+      dY[2 * (B + p) + o] = 2.f * w * rd * inv;
       l += (re * re + w * rd * rd) * inv;
     }
   }
@@ -479,6 +513,36 @@ __global__ void sobolev_loss_kernel(const float *__restrict__ Y, const float *__
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
+}
+
+// Output layer back through value and tangent rows: dH = dY W4^T on both,
+// then dZ3 = dH SiLU'(Z3) + dH~ SiLU''(Z3) Z3~ (value rows) and dZ3~ = dH~
+// SiLU'(Z3) (tangent rows, Z3 rows [B, 2B) hold Z3~) as BF16, over value
+// rows [b rpb, (b + 1) rpb), with the block's column sums of the value
+// rows' dZ3 (bias-gradient partials): part[b][j].
+__global__ __launch_bounds__(256) void out_back_jvp_kernel(const float *__restrict__ dY,
+                                                           const uint16_t *__restrict__ W4,
+                                                           const float *__restrict__ Z, int B, int rpb,
+                                                           uint16_t *__restrict__ dZb,
+                                                           float *__restrict__ part) {
+  __shared__ float red[256];
+  const int j = threadIdx.x % MLP_H, h = threadIdx.x / MLP_H;
+  const int r0 = blockIdx.x * rpb, r1 = min(B, r0 + rpb);
+  const float w0 = from_bf16(W4[2 * j]), w1 = from_bf16(W4[2 * j + 1]);
+  float cs = 0.f;
+  for (int r = r0 + h; r < r1; r += 2) {
+    const size_t i = (size_t)r * MLP_H + j, it = (size_t)(B + r) * MLP_H + j;
+    const float dh = dY[2 * r] * w0 + dY[2 * r + 1] * w1;
+    const float dht = dY[2 * (B + r)] * w0 + dY[2 * (B + r) + 1] * w1;
+    const float z = Z[i], sg = sigm(z);
+    const float dz = dh * silu1(z, sg) + dht * silu2(z, sg) * Z[it];
+    dZb[i] = to_bf16(dz);
+    dZb[it] = to_bf16(dht * silu1(z, sg));
+    cs += dz;
+  }
+  red[threadIdx.x] = cs;
+  __syncthreads();
+  if (h == 0) part[(size_t)blockIdx.x * MLP_H + j] = red[j] + red[MLP_H + j];
 }
 
 __global__ void scale_out_kernel(const float *__restrict__ Y, int n, float s0, float s1,
@@ -508,6 +572,8 @@ struct Mlp {
   int *idx = nullptr;
   double *loss = nullptr;
   float *part = nullptr;  // partial products of the split weight-gradient products
+  float *dHt = nullptr;   // tangent rows' upstream gradient of a layer (forward-mode training)
+  float *dir = nullptr;   // directions of mlp_jvp_loss_grad's points
   template <class T>
   T *alloc(size_t count) {
     void *p = nullptr;
@@ -536,12 +602,14 @@ constexpr int MAX_SPLIT = 64;  // k-chunks of a split product
 template <bool TA, bool TB, int EPI = EPI_STORE>
 void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
           float *C, int ldc, int splitk = 1, float *part = nullptr, const float *bias = nullptr,
-          const float *Zin = nullptr, uint16_t *H = nullptr) {
+          const float *Zin = nullptr, uint16_t *H = nullptr, const float *Zt = nullptr,
+          const float *Gt = nullptr) {
   if (lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15))
     throw Error(MCCS_ERR_INVALID, "matrix operand not aligned for 16-byte loads");
   if (N % 4 || ldc % 4 || ((uintptr_t)C & 15) || ((uintptr_t)H & 7))
     throw Error(MCCS_ERR_INVALID, "matrix result not aligned for vector stores");
-  if (EPI == EPI_SILU_BACK && (size_t)((M + GT - 1) / GT) * ldc > (size_t)MAX_SPLIT * MLP_H * MLP_H)
+  if ((EPI == EPI_SILU_BACK || EPI == EPI_SILU2_BACK) &&
+      (size_t)((M + GT - 1) / GT) * ldc > (size_t)MAX_SPLIT * MLP_H * MLP_H)
     throw Error(MCCS_ERR_INVALID, "column partials outside their workspace");
   int kchunk = (K + splitk - 1) / splitk;
   kchunk = (kchunk + GK - 1) / GK * GK;
@@ -549,7 +617,7 @@ void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const 
   if (!part) {
     dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, 1);
     hipLaunchKernelGGL((gemm_kernel<TA, TB, false, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
-                       C, ldc, K, bias, Zin, H);
+                       C, ldc, K, bias, Zin, H, Zt, Gt);
     return;
   }
   if (EPI != EPI_STORE || splitk > MAX_SPLIT || ldc != N)
@@ -557,7 +625,7 @@ void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const 
   dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, splitk);
   hipLaunchKernelGGL((gemm_kernel<TA, TB, true, EPI_STORE>), grid, dim3(256), 0, s, M, N, K, A, lda, B,
                      ldb, part, N, kchunk, (const float *)nullptr, (const float *)nullptr,
-                     (uint16_t *)nullptr);
+                     (uint16_t *)nullptr, (const float *)nullptr, (const float *)nullptr);
   hipLaunchKernelGGL(partsum_kernel, dim3((M * N + 3) / 4), dim3(256), 0, s, (const float *)part, splitk,
                      M * N, C);
 }
@@ -586,7 +654,34 @@ void forward(Mlp *m, int n) {
   gemm<false, false, EPI_BIAS_SILU>(s, n, MLP_H, MLP_H, m->H2, MLP_H, m->wb + MLP_W3, MLP_H, m->Z3, MLP_H,
                                     1, nullptr, m->w + MLP_B3, nullptr, m->H3);
   hipLaunchKernelGGL(out_layer_kernel, dim3(blocks_for(n)), dim3(256), 0, s, m->H3, m->wb + MLP_W4,
-                     m->w + MLP_B4, n, m->Y);
+                     m->w + MLP_B4, n, n, m->Y);
+}
+
+// Forward over B points xyz[idx[p]] (idx may be null) with the derivative
+// along dir[idx[p]]: value rows [0, B) and tangent rows [B, 2B) of X0, Z*
+// (rows [B, 2B): the tangent pre-activations), H* and Y.
+void forward_jvp(Mlp *m, int B, const float *xyz, const float *dir, const int *idx) {
+  hipStream_t s = m->stream;
+  const size_t tb = (size_t)B * MLP_H;  // tangent-row offset of a 128-wide buffer
+  hipLaunchKernelGGL(jvp_features_kernel, dim3(blocks_for((size_t)B * MLP_NF)), dim3(256), 0, s, xyz, dir,
+                     idx, B, m->ds, m->X0);
+  const uint16_t *in = m->X0;
+  size_t tin = (size_t)B * MLP_IN;
+  int K = MLP_IN;
+  const size_t wo[3] = {MLP_W1, MLP_W2, MLP_W3}, bo[3] = {MLP_B1, MLP_B2, MLP_B3};
+  float *Z[3] = {m->Z1, m->Z2, m->Z3};
+  uint16_t *H[3] = {m->H1, m->H2, m->H3};
+  for (int l = 0; l < 3; l++) {
+    gemm<false, false, EPI_BIAS_SILU>(s, B, MLP_H, K, in, K, m->wb + wo[l], MLP_H, Z[l], MLP_H, 1, nullptr,
+                                      m->w + bo[l], nullptr, H[l]);
+    gemm<false, false, EPI_TAN>(s, B, MLP_H, K, in + tin, K, m->wb + wo[l], MLP_H, Z[l] + tb, MLP_H, 1,
+                                nullptr, nullptr, Z[l], H[l] + tb);
+    in = H[l];
+    tin = tb;
+    K = MLP_H;
+  }
+  hipLaunchKernelGGL(out_layer_kernel, dim3(blocks_for(2 * (size_t)B)), dim3(256), 0, s, m->H3,
+                     m->wb + MLP_W4, m->w + MLP_B4, 2 * B, B, m->Y);
 }
 
 // Backward from m->dY; weight gradients accumulated into m->g when wgrad,
@@ -598,7 +693,7 @@ void backward(Mlp *m, int n, bool wgrad, bool xgrad) {
     // gW4 and gb4 are adjacent in the flat layout: one fixed-order reduction
     static_assert(MLP_B4 == MLP_W4 + 2 * MLP_H, "output-layer gradient layout");
     const int rpb = rows_per_block(n), nb = (n + rpb - 1) / rpb;
-    hipLaunchKernelGGL(w4part_kernel, dim3(nb), dim3(256), 0, s, m->H3, m->dY, n, rpb, m->part);
+    hipLaunchKernelGGL(w4part_kernel, dim3(nb), dim3(256), 0, s, m->H3, m->dY, n, n, rpb, m->part);
     hipLaunchKernelGGL(partsum_kernel, dim3((W4PART + 3) / 4), dim3(256), 0, s, (const float *)m->part, nb,
                        W4PART, m->g + MLP_W4);
   }
@@ -625,6 +720,47 @@ void backward(Mlp *m, int n, bool wgrad, bool xgrad) {
   }
   if (xgrad)
     gemm<false, true>(s, n, MLP_IN, MLP_H, m->dZ1b, MLP_H, m->wb + MLP_W1, MLP_H, m->dX, MLP_IN);
+}
+
+// Backward of forward_jvp from m->dY (value and tangent rows); parameter
+// gradients accumulated into m->g.  Weight gradients sum over both row
+// sets, bias gradients over the value rows.
+void backward_jvp(Mlp *m, int B) {
+  hipStream_t s = m->stream;
+  const int n2 = 2 * B;
+  const size_t tb = (size_t)B * MLP_H;
+  const int sk = std::max(1, std::min(MAX_SPLIT, n2 / 256));
+  {
+    static_assert(MLP_B4 == MLP_W4 + 2 * MLP_H, "output-layer gradient layout");
+    const int rpb = rows_per_block(n2), nb = (n2 + rpb - 1) / rpb;
+    hipLaunchKernelGGL(w4part_kernel, dim3(nb), dim3(256), 0, s, m->H3, m->dY, n2, B, rpb, m->part);
+    hipLaunchKernelGGL(partsum_kernel, dim3((W4PART + 3) / 4), dim3(256), 0, s, (const float *)m->part, nb,
+                       W4PART, m->g + MLP_W4);
+  }
+  {
+    const int rpb = rows_per_block(B), nb = (B + rpb - 1) / rpb;
+    hipLaunchKernelGGL(out_back_jvp_kernel, dim3(nb), dim3(256), 0, s, m->dY, m->wb + MLP_W4, m->Z3, B, rpb,
+                       m->dZ3b, m->part);
+    colsum(s, m->part, nb, m->g + MLP_B3);
+  }
+  const int nbt = (B + GT - 1) / GT;
+  // layers 3 -> 2 -> 1: weight gradient over both row sets, then the
+  // tangent rows' gradient (EPI_TAN: also keeps the upstream tangent gradient
+  // in dHt for the value rows), then the value rows' (EPI_SILU2_BACK)
+  const uint16_t *Hin[3] = {m->X0, m->H1, m->H2};
+  const int Kin[3] = {MLP_IN, MLP_H, MLP_H};
+  const size_t wo[3] = {MLP_W1, MLP_W2, MLP_W3}, bo[3] = {MLP_B1, MLP_B2, MLP_B3};
+  float *Z[3] = {m->Z1, m->Z2, m->Z3};
+  uint16_t *dZb[3] = {m->dZ1b, m->dZ2b, m->dZ3b};
+  for (int l = 2; l >= 0; l--) {
+    gemm<true, false>(s, Kin[l], MLP_H, n2, Hin[l], Kin[l], dZb[l], MLP_H, m->g + wo[l], MLP_H, sk, m->part);
+    if (l == 0) break;
+    gemm<false, true, EPI_TAN>(s, B, MLP_H, MLP_H, dZb[l] + tb, MLP_H, m->wb + wo[l], MLP_H, m->dHt, MLP_H, 1,
+                               nullptr, nullptr, Z[l - 1], dZb[l - 1] + tb);
+    gemm<false, true, EPI_SILU2_BACK>(s, B, MLP_H, MLP_H, dZb[l], MLP_H, m->wb + wo[l], MLP_H, m->part, MLP_H,
+                                      1, nullptr, nullptr, Z[l - 1], dZb[l - 1], Z[l - 1] + tb, m->dHt);
+    colsum(s, m->part, nbt, m->g + bo[l - 1]);
+  }
 }
 
 DSpec device_spec(const MlpSpec &sp) {
@@ -672,6 +808,8 @@ Mlp *mlp_create(int device, const MlpSpec &spec, const float *params, int max_ba
   m->idx = m->alloc<int>(B);
   m->loss = m->alloc<double>(1);
   m->part = m->alloc<float>((size_t)MAX_SPLIT * MLP_H * MLP_H);
+  m->dHt = m->alloc<float>(B / 2 * MLP_H + MLP_H);
+  m->dir = m->alloc<float>(B * 3);
   MLP_OK(hipMemcpy(m->w, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
   MLP_OK(hipMemset(m->g, 0, sizeof(float) * MLP_NPARAM));
   MLP_OK(hipMemset(m->m, 0, sizeof(float) * MLP_NPARAM));
@@ -747,11 +885,59 @@ double mlp_loss_grad(Mlp *m, const double *xyz, const double *e, uint64_t n, flo
   return loss;
 }
 
+double mlp_jvp_loss_grad(Mlp *m, const double *xyz, const double *dir, const double *e, const double *d,
+                         double w, uint64_t n, float *grad, double *y, double *yd) {
+  MLP_OK(hipSetDevice(m->device));
+  if (n == 0 || n > (uint64_t)m->maxb / 2)
+    throw Error(MCCS_ERR_INVALID, "batch larger than the network's workspace");
+  std::vector<float> p(3 * n), u(3 * n), E(2 * n), D(2 * n);
+  for (uint64_t i = 0; i < 3 * n; i++) {
+    p[i] = (float)xyz[i];
+    u[i] = (float)dir[i];
+  }
+  for (uint64_t i = 0; i < n; i++)
+    for (int o = 0; o < 2; o++) {
+      E[2 * i + o] = (float)(e[2 * i + o] / m->spec.out_scale[o]);
+      D[2 * i + o] = (float)(d[2 * i + o] / m->spec.out_scale[o]);
+    }
+  float *dE = nullptr;
+  MLP_OK(hipMalloc(&dE, sizeof(float) * 4 * n));
+  double loss = 0.0;
+  std::vector<float> Y(4 * n);
+  try {
+    MLP_OK(hipMemcpyAsync(m->pts, p.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
+    MLP_OK(hipMemcpyAsync(m->dir, u.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
+    MLP_OK(hipMemcpyAsync(dE, E.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, m->stream));
+    MLP_OK(hipMemcpyAsync(dE + 2 * n, D.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, m->stream));
+    MLP_OK(hipMemsetAsync(m->loss, 0, sizeof(double), m->stream));
+    MLP_OK(hipMemsetAsync(m->g, 0, sizeof(float) * MLP_NPARAM, m->stream));
+    forward_jvp(m, (int)n, m->pts, m->dir, nullptr);
+    MLP_OK(hipMemcpyAsync(Y.data(), m->Y, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, m->stream));
+    hipLaunchKernelGGL(jvp_loss_kernel, dim3(blocks_for(n)), dim3(256), 0, m->stream, m->Y, dE, dE + 2 * n,
+                       (const int *)nullptr, (int)n, 1.0f / (2.0f * n), (float)w, m->dY, m->loss);
+    backward_jvp(m, (int)n);
+    MLP_OK(hipMemcpyAsync(grad, m->g, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, m->stream));
+    MLP_OK(hipMemcpyAsync(&loss, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    MLP_OK(hipStreamSynchronize(m->stream));
+    MLP_OK(hipMemsetAsync(m->g, 0, sizeof(float) * MLP_NPARAM, m->stream));
+    MLP_OK(hipStreamSynchronize(m->stream));
+  } catch (...) {
+    (void)hipFree(dE);
+    throw;
+  }
+  (void)hipFree(dE);
+  for (uint64_t i = 0; i < 2 * n; i++) {
+    if (y) y[i] = Y[i];
+    if (yd) yd[i] = Y[2 * n + i];
+  }
+  return loss;
+}
+
 double mlp_train(Mlp *m, const float *d_xyz, const float *d_e, uint64_t n, const TrainOpts &o,
                  std::vector<double> *history) {
   MLP_OK(hipSetDevice(m->device));
   const bool sob = o.d_dir && o.d_deriv;
-  const int B = std::min(o.batch, sob ? m->maxb / 3 : m->maxb);
+  const int B = std::min(o.batch, sob ? m->maxb / 2 : m->maxb);
   std::mt19937_64 rng(o.seed);
   std::vector<int> perm(n);
   std::iota(perm.begin(), perm.end(), 0);
@@ -771,13 +957,10 @@ double mlp_train(Mlp *m, const float *d_xyz, const float *d_e, uint64_t n, const
         const int cnt = (int)std::min<uint64_t>(B, n - b * B);
         const int *ix = d_perm + b * B;
         if (sob) {
-          hipLaunchKernelGGL(sobolev_pts_kernel, dim3(blocks_for(cnt)), dim3(256), 0, m->stream, d_xyz,
-                             o.d_dir, ix, cnt, (float)o.fd_h, m->pts);
-          forward(m, 3 * cnt);
-          hipLaunchKernelGGL(sobolev_loss_kernel, dim3(blocks_for(cnt)), dim3(256), 0, m->stream, m->Y,
-                             d_e, o.d_deriv, ix, cnt, (float)(0.5 / o.fd_h), 1.0f / (2.0f * cnt),
-                             (float)o.w_deriv, m->dY, m->loss);
-          backward(m, 3 * cnt, true, false);
+          forward_jvp(m, cnt, d_xyz, o.d_dir, ix);
+          hipLaunchKernelGGL(jvp_loss_kernel, dim3(blocks_for(cnt)), dim3(256), 0, m->stream, m->Y, d_e,
+                             o.d_deriv, ix, cnt, 1.0f / (2.0f * cnt), (float)o.w_deriv, m->dY, m->loss);
+          backward_jvp(m, cnt);
         } else {
           hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(cnt)), dim3(256), 0, m->stream, d_xyz, ix,
                              cnt, m->pts);

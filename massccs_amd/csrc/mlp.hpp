@@ -59,12 +59,20 @@ struct TrainOpts {
   uint64_t seed = 2104;
   // optional derivative matching: per point a unit direction (float3) and
   // the scaled directional derivatives of both outputs (float2), matched by
-  // central differences of the network over +-fd_h along the direction
+  // the network's forward-mode derivative along the direction (a tangent
+  // row beside each value row, back-propagated with it), weight w_deriv
   const float *d_dir = nullptr, *d_deriv = nullptr;
-  double fd_h = 0.2, w_deriv = 1.0;
+  double w_deriv = 1.0;
 };
 double mlp_train(Mlp *m, const float *d_xyz, const float *d_e, uint64_t n, const TrainOpts &o,
                  std::vector<double> *history);
+// Loss of the values and the directional derivatives along unit vectors
+// dir (n x 3) against e and d (n x 2, kcal/mol and kcal/mol/A; scaled
+// here): sum of (y - e)^2 + w (y' - d)^2 over points and outputs / 2n, with
+// the gradient with respect to the flat parameters; y and yd (may be null)
+// receive the scaled outputs and derivatives (n x 2).  n <= max_batch / 2.
+double mlp_jvp_loss_grad(Mlp *m, const double *xyz, const double *dir, const double *e, const double *d,
+                         double w, uint64_t n, float *grad, double *y, double *yd);
 // The same from host points and energies (kcal/mol; scaled here).
 double mlp_train_host(Mlp *m, const double *xyz, const double *e, uint64_t n, const TrainOpts &o,
                       std::vector<double> *history);

@@ -114,3 +114,71 @@ def eval_with_grad(p, xyz, freq, centre, in_scale, out_scale, bf=False):
         da = dx0[:, :NF] * np.cos(c["a"]) - dx0[:, NF:] * np.sin(c["a"])
         out[:, o, :] = (da @ freq) * 2 * np.pi * in_scale * out_scale[o]
     return y * np.asarray(out_scale), out
+
+
+# ---------------------------------------------------------------- forward-mode derivative
+# The network's derivative along a direction u per point is propagated
+# forward with the values (a tangent row beside each value row):
+#   feature tangents  d sin a = cos a da,  d cos a = -sin a da,  da = 2 pi s B u
+#   layer             Z = H W + b,  dZ = dH W,  H' = SiLU(Z),  dH' = SiLU'(Z) dZ
+#   output            y = H3 W4 + b4,  dy = dH3 W4
+# Training fits y to the far field and dy to its directional derivative; the
+# loss gradient back-propagates through both rows, the value row picking up
+# SiLU''(Z) dZ times the tangent row's upstream gradient.  With bf16=True the
+# GPU's roundings are mirrored: features and their tangents, activations and
+# their tangents, and both rows' back-propagated pre-activation gradients go
+# to BF16 before a product.
+
+def silu2(z):
+    """SiLU''(z)."""
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 - s) * (2.0 + z * (1.0 - 2.0 * s))
+
+
+def jvp_forward(p, xyz, dirs, freq, centre, in_scale, bf=False):
+    """Outputs y (n, 2) and their derivatives along the unit vectors dirs
+    (n, 2; per A), in the network's scaled output units."""
+    w1, b1, w2, b2, w3, b3, w4, b4 = unpack(p)
+    freq = np.asarray(freq, dtype=np.float64)
+    x0, a = features(xyz, freq, centre, in_scale)
+    ad = 2 * np.pi * in_scale * (np.asarray(dirs, dtype=np.float64) @ freq.T)
+    xd0 = np.concatenate([np.cos(a) * ad, -np.sin(a) * ad], axis=1)
+    c = dict(x0=_r(x0, bf), xd0=_r(xd0, bf))
+    h, hd = c["x0"], c["xd0"]
+    for k, (w, b) in enumerate([(w1, b1), (w2, b2), (w3, b3)], 1):
+        z = h @ _r(w, bf) + b
+        zd = hd @ _r(w, bf)
+        hv, s1 = silu(z)
+        h, hd = _r(hv, bf), _r(s1 * zd, bf)
+        c.update({"z%d" % k: z, "zd%d" % k: zd, "h%d" % k: h, "hd%d" % k: hd,
+                  "s%d" % k: s1, "t%d" % k: silu2(z)})
+    y = h @ _r(w4, bf) + b4
+    yd = hd @ _r(w4, bf)
+    return y, yd, c
+
+
+def jvp_loss_grad(p, xyz, dirs, e, d, w, freq, centre, in_scale, out_scale, bf=False):
+    """Loss sum over points and outputs of (y - e/out_scale)^2 +
+    w (dy - d/out_scale)^2, divided by 2n, and its parameter gradient."""
+    y, yd, c = jvp_forward(p, xyz, dirs, freq, centre, in_scale, bf)
+    w1, b1, w2, b2, w3, b3, w4, b4 = unpack(p)
+    osc = np.asarray(out_scale)
+    re = y - np.asarray(e, dtype=np.float64) / osc
+    rd = yd - np.asarray(d, dtype=np.float64) / osc
+    n = len(y)
+    loss = float(((re * re).sum() + w * (rd * rd).sum()) / (2 * n))
+    dy, dyd = re / n, w * rd / n
+    g = np.zeros(NPARAM)
+    g[W4:B4] = (c["h3"].T @ dy + c["hd3"].T @ dyd).ravel()
+    g[B4:] = dy.sum(0)
+    dh, dhd = dy @ _r(w4, bf).T, dyd @ _r(w4, bf).T
+    for k, (wl, lo, hi, blo, bhi) in [(3, (w3, W3, B3, B3, W4)), (2, (w2, W2, B2, B2, W3)),
+                                      (1, (w1, W1, B1, B1, W2))]:
+        dz = dh * c["s%d" % k] + dhd * c["t%d" % k] * c["zd%d" % k]
+        dzd = dhd * c["s%d" % k]
+        g[blo:bhi] = dz.sum(0)
+        dzb, dzdb = _r(dz, bf), _r(dzd, bf)
+        hin, hdin = (c["h%d" % (k - 1)], c["hd%d" % (k - 1)]) if k > 1 else (c["x0"], c["xd0"])
+        g[lo:hi] = (hin.T @ dzb + hdin.T @ dzdb).ravel()
+        dh, dhd = dzb @ _r(wl, bf).T, dzdb @ _r(wl, bf).T
+    return loss, g, y, yd

@@ -68,6 +68,38 @@ def test_mfma_loss_gradient_matches_oracle():
         assert err <= 2e-2, (lo, hi, err)
 
 
+def test_mfma_forward_mode_loss_gradient_matches_oracle():
+    """Value and tangent rows (the outputs' derivative along a unit vector
+    per point, carried forward through the network) on the GPU against the
+    oracle's forward-mode restatement with the GPU's BF16 roundings: the
+    outputs and derivatives, the loss and every parameter block's gradient."""
+    print("case mlp jvp loss grad", flush=True)
+    p, freq, centre, s, osc, rng = _net(3)
+    n = 3001  # a partial last row tile
+    xyz = rng.uniform(-30, 30, (n, 3))
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    e = np.stack([np.sin(xyz[:, 0] / 7.0) * 2.0, np.cos(xyz[:, 1] / 5.0) * 0.5], 1)
+    d = np.stack([np.cos(xyz[:, 0] / 7.0) * 2.0 / 7.0 * dirs[:, 0],
+                  -np.sin(xyz[:, 1] / 5.0) * 0.5 / 5.0 * dirs[:, 1]], 1)
+    w = 4.0
+    spec = M.mlp_spec(freq.astype(np.float32), centre, s, osc)
+    loss, g, y, dy = M.mlp_jvp_loss_grad(spec, p, xyz, dirs, e, d, w)
+    f32 = freq.astype(np.float32).astype(np.float64)
+    x32 = xyz.astype(np.float32).astype(np.float64)
+    u32 = dirs.astype(np.float32).astype(np.float64)
+    lr, gr, yr, dyr = MO.jvp_loss_grad(p, x32, u32, e, d, w, f32, centre, s, osc, bf=True)
+    for o in range(2):
+        assert np.max(np.abs(y[:, o] - yr[:, o])) <= 1e-2 * np.sqrt(np.mean(yr[:, o] ** 2))
+        assert np.sqrt(np.mean((dy[:, o] - dyr[:, o]) ** 2)) <= 2e-2 * np.sqrt(np.mean(dyr[:, o] ** 2))
+    assert abs(loss - lr) <= 1e-3 * lr
+    blocks = [(MO.W1, MO.B1), (MO.B1, MO.W2), (MO.W2, MO.B2), (MO.B2, MO.W3), (MO.W3, MO.B3),
+              (MO.B3, MO.W4), (MO.W4, MO.B4), (MO.B4, MO.NPARAM)]
+    for lo, hi in blocks:
+        err = np.linalg.norm(g[lo:hi] - gr[lo:hi]) / max(np.linalg.norm(gr[lo:hi]), 1e-30)
+        assert err <= 2e-2, (lo, hi, err)
+
+
 def test_training_fits_a_smooth_field():
     print("case mlp train", flush=True)
     p, freq, centre, s, osc, rng = _net(2)
@@ -360,28 +392,26 @@ def test_split_site_forces_agree_across_lane_counts(workdir):
                 assert abs(U[j] - u64[j]) <= 1e-12 * scale_u, (L, site, j, U[j], u64[j])
 
 
-def _check_directional_difference(spec, params, P):
-    """Training matches the network's directional derivative by a central
-    difference over +-0.2 A.  On the trained network that difference agrees
-    with the network's analytic gradient (its backward pass) to far better
-    than the 10 % error of the network's gradient against the exact field,
-    so the difference does not limit the fit."""
+def _check_forward_mode_derivative(spec, params, P):
+    """Training fits the network's directional derivative, carried forward
+    through the network beside its value; on the trained network that
+    forward-mode derivative must agree with the reverse-mode gradient (the
+    backward pass of mccs_mlp_eval) along the same directions: two routes to
+    the same analytic derivative, apart only in BF16 rounding."""
     rng = np.random.default_rng(7)
     a = P.atoms()
     k = rng.integers(0, P.info.natoms, 4000)
     pts = np.stack([a["x"][k], a["y"][k], a["z"][k]], 1) + rng.normal(0, 12.0, (4000, 3))
     d = rng.normal(size=(4000, 3))
     d /= np.linalg.norm(d, axis=1)[:, None]
-    h = 0.2
-    y, g = M.mlp_eval(spec, params, pts)
-    yp, _ = M.mlp_eval(spec, params, pts + h * d, grad=False)
-    ym, _ = M.mlp_eval(spec, params, pts - h * d, grad=False)
-    fd = (yp - ym) / (2 * h)
+    _, g = M.mlp_eval(spec, params, pts)
+    zero = np.zeros((4000, 2))
+    _, _, _, fwd = M.mlp_jvp_loss_grad(spec, params, pts, d, zero, zero, 1.0)
+    fwd = fwd * np.array([spec.out_scale[0], spec.out_scale[1]])
     an = np.einsum("pod,pd->po", g, d)
-    rel = np.sqrt(np.mean((fd - an) ** 2)) / np.sqrt(np.mean(an ** 2))
-    print("network: central difference (h = 0.2 A) vs analytic directional derivative, rms %.2e" % rel,
-          flush=True)
-    assert rel <= 0.01
+    rel = np.sqrt(np.mean((fwd - an) ** 2)) / np.sqrt(np.mean(an ** 2))
+    print("network: forward-mode vs reverse-mode directional derivative, rms %.2e" % rel, flush=True)
+    assert rel <= 0.02
 
 
 def test_what_the_far_field_and_the_network_gain():
@@ -404,7 +434,7 @@ def test_what_the_far_field_and_the_network_gain():
                      ("network", dict(n_train=1000000, epochs=150))]:
         rep, params, spec = E.build_surrogate(shell=6.0, blend=6.0, grid=0.5, **kw)
         if name == "network":
-            _check_directional_difference(spec, params, P)
+            _check_forward_mode_derivative(spec, params, P)
         E.use_surrogate(True)
         rs = E.run(u)
         E.use_surrogate(False)

@@ -57,3 +57,50 @@ def test_bf16_rounding_is_nearest_even():
     assert r[2] == 1.0 + 2 ** -6                 # tie to even (mantissa ...1 -> up)
     assert r[3] == -2.5
     assert abs(r[4] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+def _jvp_setup(seed=4, n=30):
+    p, freq, centre, s, osc, xyz, e = _setup(seed=seed, n=n)
+    rng = np.random.default_rng(seed + 100)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    d = rng.normal(0, 0.3, (n, 2))
+    return p, freq, centre, s, osc, xyz, e, dirs, d
+
+
+def test_forward_mode_derivative_matches_finite_differences():
+    """The tangent rows carry the outputs' derivative along each point's
+    direction: equal to central differences of the outputs along it."""
+    p, freq, centre, s, osc, xyz, _, dirs, _ = _jvp_setup()
+    y, yd, _ = MO.jvp_forward(p, xyz, dirs, freq, centre, s)
+    y0, _ = MO.forward(p, xyz, freq, centre, s)
+    assert np.array_equal(y, y0)
+    h = 1e-5
+    fd = (MO.jvp_forward(p, xyz + h * dirs, dirs, freq, centre, s)[0]
+          - MO.jvp_forward(p, xyz - h * dirs, dirs, freq, centre, s)[0]) / (2 * h)
+    assert np.allclose(fd, yd, rtol=1e-6, atol=1e-9 * np.max(np.abs(yd)))
+
+
+def test_forward_mode_loss_gradient_matches_finite_differences():
+    """Gradient of the value + directional-derivative loss with respect to
+    every parameter block, against central differences of the loss; with the
+    derivative weight at zero it is the value-only loss's gradient."""
+    p, freq, centre, s, osc, xyz, e, dirs, d = _jvp_setup()
+    w = 3.0
+    loss, g, _, _ = MO.jvp_loss_grad(p, xyz, dirs, e, d, w, freq, centre, s, osc)
+    rng = np.random.default_rng(5)
+    blocks = [(MO.W1, MO.B1), (MO.B1, MO.W2), (MO.W2, MO.B2), (MO.B2, MO.W3), (MO.W3, MO.B3),
+              (MO.B3, MO.W4), (MO.W4, MO.B4), (MO.B4, MO.NPARAM)]
+    for lo, hi in blocks:
+        for i in rng.integers(lo, hi, size=3):
+            h = 1e-6
+            pp, pm = p.copy(), p.copy()
+            pp[i] += h
+            pm[i] -= h
+            fd = (MO.jvp_loss_grad(pp, xyz, dirs, e, d, w, freq, centre, s, osc)[0]
+                  - MO.jvp_loss_grad(pm, xyz, dirs, e, d, w, freq, centre, s, osc)[0]) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+    l0, g0, _, _ = MO.jvp_loss_grad(p, xyz, dirs, e, d, 0.0, freq, centre, s, osc)
+    lv, gv = MO.loss_grad(p, xyz, e, freq, centre, s, osc)
+    assert abs(l0 - lv) <= 1e-12 * lv
+    assert np.allclose(g0, gv, rtol=1e-10, atol=1e-14)
