@@ -1231,7 +1231,9 @@ __device__ __forceinline__ void rattle1(double *ri, double *rj, double *vi, doub
   const double r2 = dot3(rij, rij), rc2 = dot3(rcij, rcij), rr = dot3(rcij, rij);
   double det = rr * rr - r2 * (rc2 - d2);
   if (det < 0.0) det = 0.0;
-  const double lam = (rr - sqrt(det)) / r2;
+  // the quotient by r2 (the bond length squared, positive) through the
+  // refined hardware reciprocal: within an ulp of the division
+  const double lam = (rr - sqrt(det)) * rcp_pos(r2);
   for (int k = 0; k < 3; k++) {
     vi[k] = vci[k] - c.ci * lam * rij[k];
     vj[k] = vcj[k] + c.cj * lam * rij[k];
@@ -1252,10 +1254,13 @@ __device__ __forceinline__ void rattle2(const double *ri, const double *rj, doub
     rij[k] = ri[k] - rj[k];
     vcij[k] = vci[k] - vcj[k];
   }
-  const double lam = muij * dot3(rij, vcij) / dot3(rij, rij);
+  // quotients through reciprocals (within an ulp of the divisions; the
+  // masses are constants of the probe)
+  const double lam = muij * dot3(rij, vcij) * rcp_pos(dot3(rij, rij));
+  const double li = lam * (1.0 / mi), lj = lam * (1.0 / mj);
   for (int k = 0; k < 3; k++) {
-    vi[k] = vci[k] - lam / mi * rij[k];
-    vj[k] = vcj[k] + lam / mj * rij[k];
+    vi[k] = vci[k] - li * rij[k];
+    vj[k] = vcj[k] + lj * rij[k];
   }
 }
 
