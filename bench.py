@@ -17,7 +17,9 @@ the GPU (setup, velocity-Verlet/RATTLE, site forces, tally).  Two legs:
   table of the far field -- and mccs_tally_device.
 
 The line carries both rates, their ratio, the CCS error of the surrogate leg
-against the plain leg on the same samples with its paired standard error,
+against the plain leg on the same samples with its paired standard error
+(on the timed 10^5 and, untimed, on 4 x 10^5 trajectories of the same
+stream, which the 0.5 % criterion is judged on),
 the roofline of the surrogate leg's trajectory kernel (counted FP64 work
 over its HIP-event time) and the CPU baseline: this project's OpenMP C
 restatement of the reference path at the reference's release flags, timed
@@ -47,6 +49,17 @@ W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
 # Range-split far field of the headline leg: exact below 6 A, switched to
 # zero at 12 A, far field on a 0.5 A grid from the exact evaluator.
 SUR_SHELL, SUR_BLEND, SUR_GRID = 6.0, 6.0, 0.5
+# HBM bytes per launch of the range-split trajectory kernel on the headline
+# workload (10^5 trajectories of the 50k ellipsoid): rocprofv3 --pmc
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB per dispatch, scaled by what the
+# counters read for a 1 GiB copy: 524,300 KiB fetched, 1,048,576 KiB
+# written); profiles/traj_kernel_ellipsoid50k_split.md
+SPLIT_TRAFFIC_BYTES = (8084470 * 2.0 + 5151750 * 1.0) * 1024
+# Trajectories of the accuracy check that follows the timed steps: the
+# timed 10^5 and the next 3 x 10^5 of the same sample stream (untimed), so
+# that the paired standard error of the CCS difference (chaotic trajectories
+# make it ~0.5 % at 10^5) falls below the 0.5 % criterion
+CHECK_TRAJ = 400000
 
 
 def n2_cfg(path="memory.xyz"):
@@ -222,6 +235,18 @@ def run_headline(args, target):
     flops_s = flops_of(rs)
     err, se, cp, cs = paired_error(rs, rp)
     kern_s = ts.mean_ms()
+    # accuracy check over CHECK_TRAJ trajectories (untimed, same engines)
+    nchk = max(n, CHECK_TRAJ if not args.traj else n)
+    uc = torch.from_numpy(P.samples(nchk)).cuda()
+    rc_p = torch.zeros((nchk, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    rc_s = torch.zeros_like(rc_p)
+    E.run_device(uc.data_ptr(), nchk, rc_p.data_ptr(), sptr)
+    B.run_device([nchk], [uc.data_ptr()], [rc_s.data_ptr()], sptr)
+    torch.cuda.synchronize()
+    rcp = rc_p.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    rcs = rc_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    err4, se4, cp4, cs4 = paired_error(rcs, rcp)
+    del uc, rc_p, rc_s
     value = n * args.steps / wall_s
     plain_value = n * args.steps / wall_p
     out = {
@@ -243,7 +268,9 @@ def run_headline(args, target):
                 "paired_standard_error": round(se, 5), "device_tally_mean": float(ccs_s[0]),
                 "lost": {"surrogate": int(((rs["hit"] == 1) & (rs["success"] == 0)).sum()),
                          "plain": int(((rp["hit"] == 1) & (rp["success"] == 0)).sum())},
-                "within_0.5%": bool(abs(err) <= 0.005)},
+                "check": {"trajectories": nchk, "surrogate": cs4, "plain": cp4,
+                          "error_vs_plain": round(err4, 5), "paired_standard_error": round(se4, 5)},
+                "within_0.5%": bool(abs(err4) <= 0.005)},
         "surrogate": {"shell_A": SUR_SHELL, "switch_end_A": SUR_SHELL + SUR_BLEND,
                       "lanes_per_trajectory": args.sur_lanes,
                       "grid_A": SUR_GRID, "table": "exact far field at the nodes",
@@ -256,7 +283,8 @@ def run_headline(args, target):
         "roofline": {"bound": "fp64-valu", "achieved": round(flops_s / (kern_s * 1e-3) / 1e12, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(flops_s / (kern_s * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "traj_kernel<N2, type 6, range-split, batched>",
+                     "traffic": int(SPLIT_TRAFFIC_BYTES) if P.info.natoms == 50000 and n == 100000 else None,
+                     "kernel": "traj_kernel<N2, type 6, range-split, batched>",
                      "kernel_ms": round(kern_s, 3), "flops_per_launch": int(flops_s)},
     }
     if args.network:
@@ -265,12 +293,14 @@ def run_headline(args, target):
         rep2 = E2.build_surrogate(shell=SUR_SHELL, blend=SUR_BLEND, grid=SUR_GRID)[0]
         b2 = time.perf_counter() - t0
         E2.use_surrogate(True)
-        E2.run_device(u.data_ptr(), n, rec_s.data_ptr(), sptr)
+        uc = torch.from_numpy(P.samples(nchk)).cuda()
+        rn_d = torch.zeros((nchk, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        E2.run_device(uc.data_ptr(), nchk, rn_d.data_ptr(), sptr)
         torch.cuda.synchronize()
-        rn = rec_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
-        e2, se2, _, cn = paired_error(rn, rp)
+        rn = rn_d.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+        e2, se2, _, cn = paired_error(rn, rcp)
         out["network_table"] = {
-            "ccs_error_vs_plain": round(e2, 5), "paired_standard_error": round(se2, 5),
+            "trajectories": nchk, "ccs_error_vs_plain": round(e2, 5), "paired_standard_error": round(se2, 5),
             "build_s": round(b2, 2), "train_s": round(rep2["seconds_train"], 2),
             "net_rel_err_psi": rep2["val_rmse_u"][1] / max(rep2["val_rms_u"][1], 1e-300),
             "net_rel_err_grad": rep2["val_rmse_f"][1] / max(rep2["val_rms_f"][1], 1e-300)}
@@ -402,8 +432,8 @@ def run_batch(args):
                                  "max": int(r["steps"].max()),
                                  "mean": float(r["steps"][r["hit"] == 1].mean())},
         "autotune": {"seconds": round(t_tune, 2),
-                     "choices": [{"class": c["class"], "lanes": c["lanes"], "skin": c["skin"]}
-                                 for c in tune]},
+                     "choices": [{"class": c["class"], "lanes": c["lanes"], "skin": c["skin"],
+                                  "tile": c["tile"]} for c in tune]},
         "surrogate_build_s": build, "flops": flops_of(r), "roofline": None, "cpu_baseline": None,
     }
 

@@ -302,10 +302,12 @@ mccs_status mccs_batch_run_device(mccs_batch *b, const uint64_t *n,
                                   mccs_traj_record *const *records_dev, void *stream);
 
 /* Autotuner: for every group, times trial runs of the first n_trial[t]
- * samples of each target over lanes per trajectory {16, 32, 64} and Verlet
- * skin {0.5, 1, 2} A, keeps the fastest, and caches it for this process by
- * (probe, force type, cutoffs, target sizes).  report (may be NULL) receives
- * the trials as JSON; len the full length. */
+ * samples of each target over lanes per trajectory {16, 32, 64}, Verlet
+ * skin {0.5, 1, 2} A and LDS tile (the Verlet-list capacity: half a CU per
+ * block, two blocks per CU, or a whole CU), keeps the fastest, and caches it
+ * for this process by (probe, force type, cutoffs, target sizes, far field).
+ * report (may be NULL) receives the trials as JSON ("lanes", "skin", "tile":
+ * 1 half, 2 whole, "ms"); len the full length. */
 mccs_status mccs_batch_autotune(mccs_batch *b, const uint64_t *n_trial,
                                 const double *const *samples, char *report, size_t cap,
                                 size_t *len);

@@ -1347,6 +1347,7 @@ struct BatchClass {
   std::vector<int> members;  // target indices, in queue order
   int lanes = 64, block = 256;
   double skin = 1.0;
+  int tile = 0;              // LDS per block: 0 by the list-size rule, 1 half a CU, 2 a whole CU
   bool tuned = false;
   Tgt *d_tg = nullptr;
   unsigned long long *d_toff = nullptr;
@@ -1362,8 +1363,13 @@ struct BatchClass {
 
 // Tuned launch shapes by class signature: probe kind, force type, cutoffs and
 // the targets' sizes rounded to powers of two.
-std::map<std::string, std::pair<int, double>> &tune_cache() {
-  static std::map<std::string, std::pair<int, double>> c;
+struct Shape {
+  int lanes;
+  double skin;
+  int tile;
+};
+std::map<std::string, Shape> &tune_cache() {
+  static std::map<std::string, Shape> c;
   return c;
 }
 
@@ -1438,7 +1444,8 @@ static void configure_class(Batch &B, BatchClass &c) {
     const double rs = (c.split ? e->Ts.rlong : e->T.rlong) + c.skin + LIST_OFF_MAX;
     fit_half = fit_half && e->list_need * (rs / r1) * (rs / r1) * (rs / r1) <= 0.9 * (double)half;
   }
-  const size_t budget = fit_half ? LDS_BLOCK_HALF : LDS_BLOCK_FULL;
+  const size_t budget = c.tile == 1 ? LDS_BLOCK_HALF
+                        : (c.tile == 2 ? LDS_BLOCK_FULL : (fit_half ? LDS_BLOCK_HALF : LDS_BLOCK_FULL));
   const size_t room = base < budget ? budget - base : 0;
   const int cap = (int)(room / (entry * groups));
   std::vector<Tgt> tg;
@@ -1504,8 +1511,9 @@ Batch *batch_create(const Prepared *const *targets, int ntargets, const mccs_bat
     c.skin = o.skin > 0.0 ? o.skin : 1.0;
     auto it = tune_cache().find(class_signature(*B, c));
     if (it != tune_cache().end() && !o.lanes && !(o.skin > 0.0)) {
-      c.lanes = it->second.first;
-      c.skin = it->second.second;
+      c.lanes = it->second.lanes;
+      c.skin = it->second.skin;
+      c.tile = it->second.tile;
       c.tuned = true;
     }
     configure_class(*B, c);
@@ -1580,6 +1588,7 @@ static void run_classes(Batch &B, bool sizes_changed) {
     key.push_back((uint64_t)c.lanes);
     key.push_back((uint64_t)(c.skin * 1000.0));
     key.push_back((uint64_t)c.split);
+    key.push_back((uint64_t)c.tile);
   }
   if (!B.gexec || sizes_changed || key != B.gkey) {
     if (B.gexec) HIP_OK(hipGraphExecDestroy(B.gexec));
@@ -1669,7 +1678,9 @@ void batch_last_work(Batch *B, uint64_t out[5]) {
 }
 
 // Autotuner: per class, times trial runs (the first n_trial[t] samples of
-// every target) over lanes {16, 32, 64} x Verlet skin {0.5, 1, 2} A and
+// every target) over lanes {16, 32, 64} x Verlet skin {0.5, 1, 2} A x LDS
+// tile {half a CU, a whole CU per block} (the Verlet-list capacity: half a
+// CU runs two blocks per CU, a whole CU longer lists that overflow less) and
 // keeps the fastest; results are cached for this process by class signature.
 std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *const *samples) {
   HIP_OK(hipSetDevice(B->device));
@@ -1681,24 +1692,26 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
   for (size_t t = 0; t < nt.size(); t++) nt[t] = n_trial[t];
   bool first = true;
   for (auto &c : B->cls) {
-    std::vector<std::pair<int, double>> cand;
+    std::vector<Shape> cand;
     const bool split = B->split && [&] {
       for (int t : c.members)
         if (!B->eng[t]->sur_built) return false;
       return true;
     }();
     for (int L : lanes_c)
-      for (double sk : skin_c) {
-        if (B->lanes_opt && L != B->lanes_opt) continue;
-        if (B->skin_opt > 0.0 && sk != B->skin_opt) continue;
-        cand.push_back({L, sk});
-      }
-    if (B->skin_opt > 0.0 && cand.empty()) cand.push_back({B->lanes_opt ? B->lanes_opt : 64, B->skin_opt});
+      for (double sk : skin_c)
+        for (int tile : {1, 2}) {
+          if (B->lanes_opt && L != B->lanes_opt) continue;
+          if (B->skin_opt > 0.0 && sk != B->skin_opt) continue;
+          cand.push_back({L, sk, tile});
+        }
+    if (B->skin_opt > 0.0 && cand.empty())
+      for (int tile : {1, 2}) cand.push_back({B->lanes_opt ? B->lanes_opt : 64, B->skin_opt, tile});
     // trial samples of this class's members only
     std::vector<uint64_t> n_only(B->eng.size(), 0);
     for (int t : c.members) n_only[t] = nt[t];
     double best = 1e300;
-    std::pair<int, double> pick{c.lanes, c.skin};
+    Shape pick{c.lanes, c.skin, c.tile};
     if (!first) rep << ",";
     first = false;
     rep << "{\"class\": \"" << class_signature(*B, c) << "\", \"trials\": [";
@@ -1707,8 +1720,9 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
     HIP_OK(hipEventCreate(&e1));
     bool f2 = true;
     for (auto &cd : cand) {
-      c.lanes = cd.first;
-      c.skin = cd.second;
+      c.lanes = cd.lanes;
+      c.skin = cd.skin;
+      c.tile = cd.tile;
       float ms = 0.f;
       try {
         configure_class(*B, c);
@@ -1722,12 +1736,14 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
       } catch (const Error &e) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
-        throw Error(e.code, "autotune trial (lanes " + std::to_string(cd.first) + ", skin " +
-                                std::to_string(cd.second) + "): " + e.what());
+        throw Error(e.code, "autotune trial (lanes " + std::to_string(cd.lanes) + ", skin " +
+                                std::to_string(cd.skin) + ", tile " + std::to_string(cd.tile) +
+                                "): " + e.what());
       }
       if (!f2) rep << ", ";
       f2 = false;
-      rep << "{\"lanes\": " << cd.first << ", \"skin\": " << cd.second << ", \"ms\": " << ms << "}";
+      rep << "{\"lanes\": " << cd.lanes << ", \"skin\": " << cd.skin << ", \"tile\": " << cd.tile
+          << ", \"ms\": " << ms << "}";
       if (ms < best) {
         best = ms;
         pick = cd;
@@ -1735,12 +1751,14 @@ std::string batch_autotune(Batch *B, const uint64_t *n_trial, const double *cons
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    c.lanes = pick.first;
-    c.skin = pick.second;
+    c.lanes = pick.lanes;
+    c.skin = pick.skin;
+    c.tile = pick.tile;
     c.tuned = true;
     configure_class(*B, c);
     tune_cache()[class_signature(*B, c)] = pick;
-    rep << "], \"lanes\": " << pick.first << ", \"skin\": " << pick.second << "}";
+    rep << "], \"lanes\": " << pick.lanes << ", \"skin\": " << pick.skin << ", \"tile\": " << pick.tile
+        << "}";
   }
   rep << "]";
   // the graph (if any) was captured with the old shapes
@@ -1774,10 +1792,12 @@ void batch_use_surrogate(Batch *B, bool on) {
     // tuned shapes are cached per signature, which names the far field
     auto it = tune_cache().find(class_signature(*B, c));
     if (it != tune_cache().end() && !B->lanes_opt && !(B->skin_opt > 0.0)) {
-      c.lanes = it->second.first;
-      c.skin = it->second.second;
-    } else if (!B->lanes_opt) {
-      c.lanes = 64;
+      c.lanes = it->second.lanes;
+      c.skin = it->second.skin;
+      c.tile = it->second.tile;
+    } else {
+      if (!B->lanes_opt) c.lanes = 64;
+      c.tile = 0;
     }
     configure_class(*B, c);
   }

@@ -85,7 +85,12 @@ def measure(P, cfg, n_iter, n_traj, threads=None, repeats=5, var=None):
     threads = threads or host_cores()
     var = var or variant()
     orc = O.Oracle(oracle_prep(P), cfg, variant=var)
-    runs = [ccs_time(orc, n_iter, n_traj, threads) for _ in range(repeats)]
+    runs = []
+    for k in range(repeats):
+        # progress on stderr: a long CPU run is not mistaken for a hang
+        print("cpu baseline: run %d of %d, %d trajectories on %d threads"
+              % (k + 1, repeats, n_iter * n_traj, threads), file=sys.stderr, flush=True)
+        runs.append(ccs_time(orc, n_iter, n_traj, threads))
     t = statistics.median(r[0] for r in runs)
     return {"seconds": t, "all_seconds": [r[0] for r in runs], "omega": runs[0][1].tolist(),
             "mean": runs[0][2], "err": runs[0][3], "threads": threads, "variant": var,
@@ -99,7 +104,8 @@ def baseline(P, cfg, budget_s=15.0, repeats=5, threads=None):
     threads = threads or host_cores()
     var = variant()
     orc = O.Oracle(oracle_prep(P), cfg, variant=var)
-    n0 = 32 * threads
+    n0 = 4 * threads
+    print("cpu baseline: probe of %d trajectories" % n0, file=sys.stderr, flush=True)
     t0 = ccs_time(orc, 1, n0, threads)[0]
     n = int(min(10 ** 6, max(n0, n0 * budget_s / max(t0, 1e-3))))
     m = measure(P, cfg, 1, n, threads=threads, repeats=repeats, var=var)

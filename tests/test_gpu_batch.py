@@ -48,10 +48,12 @@ def test_batch_autotune_and_parity(workdir):
     rep = B.autotune([u[: max(64, len(u) // 4)] for u in us])
     assert len(rep) == 2  # He type 4 and N2 type 6
     for c in rep:
-        assert len(c["trials"]) == 9 and c["lanes"] in (16, 32, 64) and c["skin"] in (0.5, 1.0, 2.0)
+        # lanes x skin x LDS tile (half or whole CU per block)
+        assert len(c["trials"]) == 18 and c["lanes"] in (16, 32, 64) and c["skin"] in (0.5, 1.0, 2.0)
+        assert c["tile"] in (1, 2) and {t["tile"] for t in c["trials"]} == {1, 2}
         best = min(t["ms"] for t in c["trials"])
         assert any(t["ms"] == best and t["lanes"] == c["lanes"] and t["skin"] == c["skin"]
-                   for t in c["trials"])
+                   and t["tile"] == c["tile"] for t in c["trials"])
     lanes, skin = B.shapes()
     assert set(lanes) <= {16, 32, 64}
     recs = B.run(us)
