@@ -165,6 +165,9 @@ class Timer:
     def mean_ms(self):
         return float(np.mean([a.elapsed_time(b) for a, b in self.ev]))
 
+    def total_ms(self):
+        return float(np.sum([a.elapsed_time(b) for a, b in self.ev]))
+
 
 def run_headline(args, target):
     """Plain vs surrogate+batched on one large N2 target (samples in HBM)."""
@@ -172,6 +175,13 @@ def run_headline(args, target):
     import massccs_amd as M
     cfg, (s, xyz, q), label = target()
     n = args.traj or 100000
+    # a step may run as several launches of `chunk` trajectories (10^6-trajectory
+    # runs of the large shell), each followed by a progress line on stderr
+    chunk = args.chunk or n
+    if n % chunk:
+        sys.exit("--chunk must divide the number of trajectories")
+    starts = list(range(0, n, chunk))
+    RS = M.RECORD_DTYPE.itemsize
     P = M.Prepared(M.config_from_dict(cfg), symbols=list(s), xyz=xyz, q=q)
     stream = torch.cuda.Stream()  # default priority; every launch and event goes on it
     torch.cuda.set_stream(stream)
@@ -186,12 +196,20 @@ def run_headline(args, target):
     # plain leg
     E = M.Engine(P, count_work=True)
 
-    def plain_step(timer=None):
-        if timer:
-            with timer:
-                E.run_device(u.data_ptr(), n, rec_p.data_ptr(), sptr)
-        else:
-            E.run_device(u.data_ptr(), n, rec_p.data_ptr(), sptr)
+    def launches(run, rec, timer, leg, first_only=False):
+        for k, a in enumerate(starts[:1] if first_only else starts):
+            if timer:
+                with timer:
+                    run(u.data_ptr() + 72 * a, rec.data_ptr() + RS * a)
+            else:
+                run(u.data_ptr() + 72 * a, rec.data_ptr() + RS * a)
+            if len(starts) > 1:
+                torch.cuda.synchronize()
+                print("%s leg: launch %d of %d done" % (leg, k + 1, len(starts)), file=sys.stderr,
+                      flush=True)
+
+    def plain_step(timer=None, first_only=False):
+        launches(lambda up, rp: E.run_device(up, chunk, rp, sptr), rec_p, timer, "plain", first_only)
         E.tally_device(rec_p.data_ptr(), 1, n, om.data_ptr(), cnt.data_ptr(), me.data_ptr(), sptr)
 
     for _ in range(args.warmup):
@@ -213,16 +231,13 @@ def run_headline(args, target):
     build_s = time.perf_counter() - t0
     B.use_surrogate(True)
 
-    def sur_step(timer=None):
-        if timer:
-            with timer:
-                B.run_device([n], [u.data_ptr()], [rec_s.data_ptr()], sptr)
-        else:
-            B.run_device([n], [u.data_ptr()], [rec_s.data_ptr()], sptr)
+    def sur_step(timer=None, first_only=False):
+        launches(lambda up, rp: B.run_device([chunk], [up], [rp], sptr), rec_s, timer, "surrogate",
+                 first_only)
         E.tally_device(rec_s.data_ptr(), 1, n, om.data_ptr(), cnt.data_ptr(), me.data_ptr(), sptr)
 
-    for _ in range(max(1, args.warmup)):  # the first run captures the graph
-        sur_step()
+    for w in range(max(1, args.warmup)):  # the first launch captures the graph
+        sur_step(first_only=args.warmup == 0)
     torch.cuda.synchronize()
     ts = Timer(stream)
     t0 = time.perf_counter()
@@ -234,19 +249,25 @@ def run_headline(args, target):
     rs = rec_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
     flops_s = flops_of(rs)
     err, se, cp, cs = paired_error(rs, rp)
-    kern_s = ts.mean_ms()
-    # accuracy check over CHECK_TRAJ trajectories (untimed, same engines)
-    nchk = max(n, CHECK_TRAJ if not args.traj else n)
-    uc = torch.from_numpy(P.samples(nchk)).cuda()
-    rc_p = torch.zeros((nchk, M.RECORD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-    rc_s = torch.zeros_like(rc_p)
-    E.run_device(uc.data_ptr(), nchk, rc_p.data_ptr(), sptr)
-    B.run_device([nchk], [uc.data_ptr()], [rc_s.data_ptr()], sptr)
-    torch.cuda.synchronize()
-    rcp = rc_p.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
-    rcs = rc_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+    # kernel time per step (the sum of its launches) and per launch
+    kern_s = ts.total_ms() / args.steps
+    kern_p = tp.total_ms() / args.steps
+    # accuracy check over CHECK_TRAJ trajectories (untimed, same engines);
+    # a larger timed run is its own check
+    nchk = n if n >= CHECK_TRAJ or args.traj else CHECK_TRAJ
+    if nchk == n:
+        rcp, rcs = rp, rs
+    else:
+        uc = torch.from_numpy(P.samples(nchk)).cuda()
+        rc_p = torch.zeros((nchk, RS), dtype=torch.uint8, device="cuda")
+        rc_s = torch.zeros_like(rc_p)
+        E.run_device(uc.data_ptr(), nchk, rc_p.data_ptr(), sptr)
+        B.run_device([nchk], [uc.data_ptr()], [rc_s.data_ptr()], sptr)
+        torch.cuda.synchronize()
+        rcp = rc_p.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+        rcs = rc_s.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
+        del uc, rc_p, rc_s
     err4, se4, cp4, cs4 = paired_error(rcs, rcp)
-    del uc, rc_p, rc_s
     value = n * args.steps / wall_s
     plain_value = n * args.steps / wall_p
     out = {
@@ -260,9 +281,9 @@ def run_headline(args, target):
                    "n_atoms": P.info.natoms, "trajectories_per_step": n, "probe": "N2",
                    "force_type": P.info.force_type, "parallelism": "1 GPU"},
         "plain": {"value": round(plain_value, 1), "ms_per_step": round(1e3 * wall_p / args.steps, 3),
-                  "kernel_ms": round(tp.mean_ms(), 3), "ccs": cp,
-                  "force_evals_per_s": round(3 * int(rp["steps"].sum()) / (tp.mean_ms() * 1e-3), 1),
-                  "roofline_frac": round(flops_p / (tp.mean_ms() * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)},
+                  "kernel_ms": round(kern_p, 3), "launches_per_step": len(starts), "ccs": cp,
+                  "force_evals_per_s": round(3 * int(rp["steps"].sum()) / (kern_p * 1e-3), 1),
+                  "roofline_frac": round(flops_p / (kern_p * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)},
         "speedup_vs_plain": round(value / plain_value, 3),
         "ccs": {"surrogate": cs, "plain": cp, "error_vs_plain": round(err, 5),
                 "paired_standard_error": round(se, 5), "device_tally_mean": float(ccs_s[0]),
@@ -283,9 +304,10 @@ def run_headline(args, target):
         "roofline": {"bound": "fp64-valu", "achieved": round(flops_s / (kern_s * 1e-3) / 1e12, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(flops_s / (kern_s * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-                     "traffic": int(SPLIT_TRAFFIC_BYTES) if P.info.natoms == 50000 and n == 100000 else None,
+                     "traffic": int(SPLIT_TRAFFIC_BYTES) if P.info.natoms == 50000 and chunk == 100000 else None,
                      "kernel": "traj_kernel<N2, type 6, range-split, batched>",
-                     "kernel_ms": round(kern_s, 3), "flops_per_launch": int(flops_s)},
+                     "kernel_ms": round(kern_s / len(starts), 3), "launches_per_step": len(starts),
+                     "flops_per_launch": int(flops_s / len(starts))},
     }
     if args.network:
         E2 = M.Engine(P, count_work=True)
@@ -502,6 +524,8 @@ def main():
                     help="ellipsoid50k: the headline configuration (plain vs surrogate+batched); "
                          "the others are secondary measurements")
     ap.add_argument("--traj", type=int, default=0, help="trajectories (per target for batch64)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="headline workloads: trajectories per launch (must divide --traj; 0 = all)")
     ap.add_argument("--network", action="store_true",
                     help="headline workloads: also build the far field from the trained network "
                          "and report its CCS error")

@@ -2015,9 +2015,13 @@ static __global__ void queue_init(Queue Q, uint64_t n) {
 // parameter and a slow speed are the likely long ones (they reach the
 // target's core and orbit), so primary tickets take them first: a counting
 // sort of the trajectories by key = (b / bmax)^2 u_v (u_v the speed draw,
-// whose speed grows with it) into ORDER_BUCKETS buckets, ascending.  The
-// order inside a bucket is that of the atomics: any order gives the same
-// records.
+// whose speed grows with it) into ORDER_BUCKETS buckets, ascending.  Over
+// several targets (a batch class, its targets largest first) the buckets
+// are shared out over the targets, which keep their order: a launch works
+// through one target's atoms at a time (mixing the targets measured 14 %
+// slower on the 64-assembly batch, the atoms of many targets competing for
+// L2), each target's trajectories in the start order.  The order inside a
+// bucket is that of the atomics: any order gives the same records.
 
 __device__ __forceinline__ int order_bucket(const Tgt *tg, const unsigned long long *toff, int ntg,
                                             const double *u, uint64_t j) {
@@ -2029,6 +2033,12 @@ __device__ __forceinline__ int order_bucket(const Tgt *tg, const unsigned long l
   }
   const double x = u[9 * j] / tg[lo].bmax, v = u[9 * j + 1];
   const double key = x * x * v;  // in [0, 1] for b in [0, bmax]; NaN-safe below
+  if (ntg > 1) {  // the target's share of the buckets, in target order
+    const int per = ORDER_BUCKETS / ntg;
+    if (per < 2) return (int)(((long long)lo * ORDER_BUCKETS) / ntg);
+    const int b = (int)(key * per);
+    return lo * per + (b < 0 ? 0 : (b >= per ? per - 1 : b));
+  }
   const int b = (int)(key * ORDER_BUCKETS);
   return b < 0 ? 0 : (b >= ORDER_BUCKETS ? ORDER_BUCKETS - 1 : b);
 }
