@@ -1385,6 +1385,13 @@ constexpr int ORDER_BUCKETS = 1024;
 
 // A first try that has run this many steps is likely to orbit and restart.
 constexpr int SPAWN_STEPS = 16384;
+// A trajectory whose tries have failed this many times is one of the long
+// orbiting chains (on the 50k ellipsoid: 0.4 % of the trajectories, each up
+// to 3.5 x 10^5 steps over its nine tries, longer than a group's whole share
+// of the launch): its remaining tries are spawned at once, whatever the
+// stage of the launch, and groups take spawned tries before new trajectories,
+// so that the chain's tries run side by side instead of one after another.
+constexpr int SPAWN_FAILS = 4;
 
 __device__ __forceinline__ unsigned long long ld_acq(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1593,7 +1600,41 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
         got = true;
       }
       while (!got && !wait && !done) {
-        // 1. the next primary trajectory (the loop only repeats past misses)
+        // 1. a spawned try (the slot taken earlier, or the next one): spawned
+        // tries are the long chains', which start before new trajectories
+        if (Q.spec_below > 0) {
+          unsigned long long slot = ~0ull;
+          if (lane == 0) {
+            if (st.has_slot) {
+              slot = st.slot;
+            } else if (ld_rlx(Q.shead) < ld_rlx(Q.stail)) {
+              slot = atomicAdd(Q.shead, 1ull);
+            }
+          }
+          slot = bcast<L>(slot);
+          if (slot != ~0ull) {
+            unsigned long long item = ~0ull;
+            int moot = 0;
+            if (lane == 0) {
+              item = slot < Q.nslots ? ld_acq(Q.slots + slot) : ~0ull;
+              st.has_slot = item == ~0ull;  // written soon by its spawner, or never
+              st.slot = slot;
+              if (item != ~0ull) moot = ld_rlx(Q.resolved + (item >> 4));
+            }
+            item = bcast<L>(item);
+            moot = __shfl(moot, 0, L);
+            if (item != ~0ull) {
+              if (moot) continue;  // the trajectory is resolved: take the next one
+              j = item >> 4;
+              k = (int)(item & 15);
+              tj = target_of(j);
+              setup<S>(tg[tj], u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
+              got = true;
+              break;
+            }
+          }
+        }
+        // 2. the next primary trajectory (the loop only repeats past misses)
         unsigned long long t = 0;
         if (lane == 0) {  // look before taking: idle groups must not hammer the counter
           t = __hip_atomic_load(Q.ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1613,40 +1654,9 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
           }
           continue;
         }
-        // 2. a spawned try (the slot taken earlier, or the next one)
         if (Q.spec_below <= 0) {  // nothing is ever spawned: this group is finished
           done = true;
           break;
-        }
-        unsigned long long slot = ~0ull;
-        if (lane == 0) {
-          if (st.has_slot) {
-            slot = st.slot;
-          } else if (ld_rlx(Q.shead) < ld_rlx(Q.stail)) {
-            slot = atomicAdd(Q.shead, 1ull);
-          }
-        }
-        slot = bcast<L>(slot);
-        if (slot != ~0ull) {
-          unsigned long long item = ~0ull;
-          int moot = 0;
-          if (lane == 0) {
-            item = slot < Q.nslots ? ld_acq(Q.slots + slot) : ~0ull;
-            st.has_slot = item == ~0ull;  // written soon by its spawner, or never
-            st.slot = slot;
-            if (item != ~0ull) moot = ld_rlx(Q.resolved + (item >> 4));
-          }
-          item = bcast<L>(item);
-          moot = __shfl(moot, 0, L);
-          if (item != ~0ull) {
-            if (moot) continue;  // the trajectory is resolved: take the next one
-            j = item >> 4;
-            k = (int)(item & 15);
-            tj = target_of(j);
-            setup<S>(tg[tj], u + 9 * j, pos, vel, rcm, vcm);  // a trajectory with tries hits
-            got = true;
-            break;
-          }
         }
         // 3. nothing to take: finished once every trajectory is resolved
         long long act = 0;
@@ -1978,7 +1988,8 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
             st.k = k + 1;
             st.cont = 1;
           }
-          if (ld_rlx(Q.active) < Q.spec_below) spawn_rest(Q, j);
+          if (ld_rlx(Q.active) < Q.spec_below || (Q.spec_below > 0 && k + 1 >= SPAWN_FAILS))
+            spawn_rest(Q, j);
         }
       }
     }
