@@ -1901,6 +1901,26 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
           vcm[d] = 0.5 * (vi[d] + vj[d]);
         }
       }
+      if constexpr (SPLIT) {
+        // the far-field coefficients of the next evaluation's sites, loaded
+        // into the cache now so that their latency (gathers that mostly
+        // miss L2) passes during the state stores, the loop and the list
+        // check instead of stalling the next pair loop
+        double pn[S::NS][3];
+        constexpr int mid = GK == 2 ? 2 : 1;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          if (S::NS == 1) {
+            pn[0][d] = ri[d];
+          } else {
+            pn[PA][d] = ri[d];
+            pn[PB][d] = rj[d];
+            pn[mid][d] = rn[d];
+          }
+        }
+        FarNode<64 / L> fnn[S::NS];
+        far_load<S, L>(T, lane, pn, fnn, -1, &fcache);
+      }
       double na;
       const double dth = st.dtheta + anglevec_nb(rn, rcm, st.rnorm, na);
       lds_fence();
