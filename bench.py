@@ -54,7 +54,7 @@ SUR_SHELL, SUR_BLEND, SUR_GRID = 6.0, 6.0, 0.5
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB per dispatch, scaled by what the
 # counters read for a 1 GiB copy: 524,300 KiB fetched, 1,048,576 KiB
 # written); profiles/traj_kernel_ellipsoid50k_split.md
-SPLIT_TRAFFIC_BYTES = (8084470 * 2.0 + 5151750 * 1.0) * 1024
+SPLIT_TRAFFIC_BYTES = (8819080 * 2.0 + 1230526 * 1.0) * 1024
 # Trajectories of the accuracy check that follows the timed steps: the
 # timed 10^5 and the next 3 x 10^5 of the same sample stream (untimed), so
 # that the paired standard error of the CCS difference (chaotic trajectories

@@ -1393,6 +1393,45 @@ constexpr int SPAWN_STEPS = 16384;
 // so that the chain's tries run side by side instead of one after another.
 constexpr int SPAWN_FAILS = 4;
 
+// The launch ends when its longest chain of tries does, and that chain runs
+// at the latency of one step while the other waves of its SIMD compete for
+// the same issue slots.  A whole-wave group raises its wave's issue priority
+// with the try's index (a restarted trajectory is more likely one of the long
+// chains; a spawned late try is one) and as the try grows long, so the long
+// tries issue first and the short ones fill the remaining cycles; the
+// priority is dropped when the try ends.  Only the order of issue changes.
+constexpr int PRIO_STEPS1 = 4096, PRIO_STEPS2 = 16384, PRIO_STEPS3 = 65536;
+template <int L>
+__device__ __forceinline__ void set_prio(int level) {
+  if constexpr (L == 64) {
+    const int v = __builtin_amdgcn_readfirstlane(level);
+    if (v <= 0) __builtin_amdgcn_s_setprio(0);
+    else if (v == 1) __builtin_amdgcn_s_setprio(1);
+    else if (v == 2) __builtin_amdgcn_s_setprio(2);
+    else __builtin_amdgcn_s_setprio(3);
+  }
+}
+__device__ __forceinline__ int prio_of_try(int k) { return k >= SPAWN_FAILS ? 2 : (k >= 1 ? 1 : 0); }
+template <int L>
+__device__ __forceinline__ void prio_try_start(int k) {
+  set_prio<L>(prio_of_try(k));
+}
+template <int L>
+__device__ __forceinline__ void prio_step(int steps, int k) {
+  if constexpr (L == 64) {
+    const int s = __builtin_amdgcn_readfirstlane(steps);
+    if (s == PRIO_STEPS1 || s == PRIO_STEPS2 || s == PRIO_STEPS3) {
+      const int lv = s == PRIO_STEPS1 ? 1 : (s == PRIO_STEPS2 ? 2 : 3);
+      const int base = prio_of_try(k);
+      set_prio<L>(lv > base ? lv : base);
+    }
+  }
+}
+template <int L>
+__device__ __forceinline__ void prio_try_end() {
+  if constexpr (L == 64) __builtin_amdgcn_s_setprio(0);
+}
+
 __device__ __forceinline__ unsigned long long ld_acq(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1710,6 +1749,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
         st.k = k;
         st.cont = 0;
       }
+      prio_try_start<L>(k);
       status = ST_INIT;
     } else if (status == ST_STEP) {  // first half (the first step of a try)
       const Tgt &T = tg[cur()];
@@ -1886,6 +1926,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
       if (__shfl(moot, 0, L)) code = TRY_ABANDON;
     }
     if (code == TRY_PENDING) {
+      prio_step<L>(steps, k);
       // the first half of the next step, from the registers
       double rn[3];
       if (S::NS == 1) {
@@ -1953,6 +1994,7 @@ __global__ __launch_bounds__(traj_block(LDS)) __attribute__((amdgpu_waves_per_eu
     }
     lds_fence();
     // the try has ended
+    prio_try_end<L>();
     uint32_t wv = 0, wl = 0, wc = 0, wd = 0;
     if (COUNT) {
       wv = wtotal<L>(w.visits);
