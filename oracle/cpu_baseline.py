@@ -104,10 +104,18 @@ def baseline(P, cfg, budget_s=15.0, repeats=5, threads=None):
     threads = threads or host_cores()
     var = variant()
     orc = O.Oracle(oracle_prep(P), cfg, variant=var)
+    # A small prefix is timed by its longest trajectory (the threads idle
+    # while one orbiting trajectory finishes), which would make the rate
+    # depend on the prefix: the size is estimated twice, from a probe and
+    # then from a first run of the estimate, and is at least 32 trajectories
+    # per thread.
     n0 = 4 * threads
     print("cpu baseline: probe of %d trajectories" % n0, file=sys.stderr, flush=True)
     t0 = ccs_time(orc, 1, n0, threads)[0]
-    n = int(min(10 ** 6, max(n0, n0 * budget_s / max(t0, 1e-3))))
+    n1 = int(min(10 ** 6, max(32 * threads, n0 * budget_s / max(t0, 1e-3))))
+    print("cpu baseline: sizing run of %d trajectories" % n1, file=sys.stderr, flush=True)
+    t1 = ccs_time(orc, 1, n1, threads)[0]
+    n = int(min(10 ** 6, max(32 * threads, n1 * budget_s / max(t1, 1e-3))))
     m = measure(P, cfg, 1, n, threads=threads, repeats=repeats, var=var)
     return {"value": n / m["seconds"], "unit": "trajectories/s", "cores": threads, "kind": "port",
             "sample": "first %d trajectories of the same seeded sample stream (nIter 1); "
