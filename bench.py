@@ -224,12 +224,24 @@ def run_headline(args, target):
     rp = rec_p.cpu().numpy().reshape(-1).view(M.RECORD_DTYPE)
     flops_p = flops_of(rp)
 
-    # surrogate+batched leg: the target as a batch of one, range-split far field
-    B = M.Batch([P], lanes=args.sur_lanes, count_work=True, use_graph=True, skin=1.0)
+    # surrogate+batched leg: the target as a batch of one, range-split far
+    # field; its launch shape (lanes per trajectory, Verlet skin, LDS tile)
+    # picked by the batch autotuner from trial runs over one step's samples
+    # (untimed), unless --sur-lanes fixes the lanes (skin 1 A)
+    tuned = args.sur_lanes == 0
+    B = M.Batch([P], lanes=args.sur_lanes, count_work=True, use_graph=True,
+                skin=0.0 if tuned else 1.0)
     t0 = time.perf_counter()
     rep = B.build_surrogate(shell=SUR_SHELL, blend=SUR_BLEND, grid=SUR_GRID, exact_table=True)[0]
     build_s = time.perf_counter() - t0
     B.use_surrogate(True)
+    shape = {"lanes": args.sur_lanes, "skin": 1.0, "tile": None, "autotune_s": None}
+    if tuned:
+        t0 = time.perf_counter()
+        tune = B.autotune([P.samples(chunk)])[0]
+        shape = {"lanes": tune["lanes"], "skin": tune["skin"], "tile": tune["tile"],
+                 "autotune_s": round(time.perf_counter() - t0, 2), "trials": len(tune["trials"])}
+        print("autotuned range-split shape: %s" % shape, file=sys.stderr, flush=True)
 
     def sur_step(timer=None, first_only=False):
         launches(lambda up, rp: B.run_device([chunk], [up], [rp], sptr), rec_s, timer, "surrogate",
@@ -293,7 +305,8 @@ def run_headline(args, target):
                           "error_vs_plain": round(err4, 5), "paired_standard_error": round(se4, 5)},
                 "within_0.5%": bool(abs(err4) <= 0.005)},
         "surrogate": {"shell_A": SUR_SHELL, "switch_end_A": SUR_SHELL + SUR_BLEND,
-                      "lanes_per_trajectory": args.sur_lanes,
+                      "lanes_per_trajectory": shape["lanes"], "skin_A": shape["skin"],
+                      "lds_tile": shape["tile"], "autotune_s": shape["autotune_s"],
                       "grid_A": SUR_GRID, "table": "exact far field at the nodes",
                       "grid_nodes": rep["grid_nodes"], "build_s": round(build_s, 2),
                       "table_rel_err_psi": rep["val_rmse_u"][0] / max(rep["val_rms_u"][0], 1e-300),
@@ -531,8 +544,9 @@ def main():
                          "and report its CCS error")
     ap.add_argument("--surrogate", action="store_true", help="batch64: range-split far fields")
     ap.add_argument("--epochs", type=int, default=300, help="network: training epochs")
-    ap.add_argument("--sur-lanes", type=int, default=64, choices=[16, 32, 64],
-                    help="headline workloads: lanes per trajectory of the range-split leg")
+    ap.add_argument("--sur-lanes", type=int, default=0, choices=[0, 16, 32, 64],
+                    help="headline workloads: lanes per trajectory of the range-split leg "
+                         "(0: the batch autotuner picks lanes, skin and LDS tile)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB float32 tensor three times: the calibration of "
                          "the FETCH_SIZE / WRITE_SIZE counters for the traffic figure")
