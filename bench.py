@@ -50,11 +50,12 @@ W_VISIT, W_LJ, W_COUL, W_DIP, W_EPI = 9, 18, 19, 39, 27
 # zero at 12 A, far field on a 0.5 A grid from the exact evaluator.
 SUR_SHELL, SUR_BLEND, SUR_GRID = 6.0, 6.0, 0.5
 # HBM bytes per launch of the range-split trajectory kernel on the headline
-# workload (10^5 trajectories of the 50k ellipsoid): rocprofv3 --pmc
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB per dispatch, scaled by what the
-# counters read for a 1 GiB copy: 524,300 KiB fetched, 1,048,576 KiB
-# written); profiles/traj_kernel_ellipsoid50k_split.md
-SPLIT_TRAFFIC_BYTES = (8819080 * 2.0 + 1230526 * 1.0) * 1024
+# workload (10^5 trajectories of the 50k ellipsoid, the autotuned shape: 64
+# lanes, skin 0.5 A, half-CU tile): rocprofv3 --pmc FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (KiB per dispatch, scaled by what the counters read for a
+# 1 GiB copy: 524,300 KiB fetched, 1,048,576 KiB written);
+# profiles/traj_kernel_ellipsoid50k_split.md
+SPLIT_TRAFFIC_BYTES = (8718449 * 2.0 + 1230772 * 1.0) * 1024
 # Trajectories of the accuracy check that follows the timed steps: the
 # timed 10^5 and the next 3 x 10^5 of the same sample stream (untimed), so
 # that the paired standard error of the CCS difference (chaotic trajectories
