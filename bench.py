@@ -14,7 +14,9 @@ the GPU (setup, velocity-Verlet/RATTLE, site forces, tally).  Two legs:
   (mccs_batch_run_device: the persistent launch replayed from its hipGraph)
   with the range-split far field -- pairs nearer than 6 A exact, the
   remainder of the electrostatic terms beyond them from a cubic B-spline
-  table of the far field -- and mccs_tally_device.
+  table of the far field -- and mccs_tally_device, in the launch shape
+  (lanes per trajectory, Verlet skin, LDS tile) that mccs_batch_autotune
+  picks from trial runs over the step's samples before the timed steps.
 
 The line carries both rates, their ratio, the CCS error of the surrogate leg
 against the plain leg on the same samples with its paired standard error
