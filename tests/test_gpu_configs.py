@@ -78,6 +78,32 @@ def test_large_target_trajectories_match_oracle(which, n, min_big):
     assert_traj_parity(rec, refs, which, min_big=min_big)
 
 
+def test_ellipsoid_ccs_and_losses_agree_with_oracle():
+    """A statistical check of the 50k ellipsoid (N2, force type 6) against the
+    oracle on the same 2,000 samples: the CCS within three paired standard
+    errors (chaotic trajectories part ways on any rounding change, so the
+    records of the two cannot agree one by one), and the number of
+    trajectories lost to the reference's energy test within four Poisson
+    standard deviations (both lose about 8 %)."""
+    n = 2000
+    print("case ccs ellipsoid50k vs oracle, %d trajectories" % n, flush=True)
+    P = _target("ellipsoid50k")
+    u = P.samples(n)
+    rec = M.Engine(P).run(u)
+    ref = oracle_for(P, N2_CFG).run(u)
+    assert np.array_equal(rec["hit"], ref["hit"])
+    den = float(((ref["hit"] == 0) | (ref["success"] == 1)).sum())
+    dens = float(((rec["hit"] == 0) | (rec["success"] == 1)).sum())
+    co, cg = ref["domega"].sum() / den, rec["domega"].sum() / dens
+    se = float(np.std(rec["domega"] - ref["domega"]) * np.sqrt(n) / den)
+    lost_o = int(((ref["hit"] == 1) & (ref["success"] == 0)).sum())
+    lost_g = int(((rec["hit"] == 1) & (rec["success"] == 0)).sum())
+    print("ccs gpu %.2f oracle %.2f paired SE %.2f; lost gpu %d oracle %d"
+          % (cg, co, se, lost_g, lost_o), flush=True)
+    assert abs(cg - co) <= 3 * se + 1e-9 * co
+    assert abs(lost_g - lost_o) <= 4 * np.sqrt(max(lost_o, 1)) + 1
+
+
 def test_shell_ccs_independent_of_summation_order():
     """2 x 10^4 trajectories of the 10^6-atom shell with 64 and with 32 lanes
     per trajectory (the same force field, summed in another order): hit flags
