@@ -315,7 +315,10 @@ def run_headline(args, target):
                       "table_rel_err_psi": rep["val_rmse_u"][0] / max(rep["val_rms_u"][0], 1e-300),
                       "table_rel_err_grad": rep["val_rmse_f"][0] / max(rep["val_rms_f"][0], 1e-300),
                       "visits_ratio": round(float(rs["visits"].sum()) / max(1.0, float(rp["visits"].sum())), 4),
-                      "note": "value excludes the one-off table build (build_s)"},
+                      "note": "value excludes the one-off table build (build_s) and, in both "
+                              "legs, the host generation of the samples (the reference's CCS "
+                              "time includes it: about 1 ms per 10^5 samples here, resident "
+                              "in HBM before the timed steps)"},
         "force_evals_per_s": round(3 * int(rs["steps"].sum()) / (kern_s * 1e-3), 1),
         "roofline": {"bound": "fp64-valu", "achieved": round(flops_s / (kern_s * 1e-3) / 1e12, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

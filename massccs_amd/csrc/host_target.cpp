@@ -481,11 +481,6 @@ std::string fmt_time(double s) {
 void speed_table(Prepared &P) {
   const double m = P.mu * AMU_TO_KG;
   const double T = P.cfg.temperature;
-  auto f = [&](double v) {
-    double vs = v * ANG_TO_M / FS_TO_S;
-    return std::pow(vs, 5.0) * std::pow((m / (2.0 * BOLTZMANN_K * T)), 3.0) *
-           std::exp(-(m * std::pow(vs, 2.0)) / (2.0 * BOLTZMANN_K * T));
-  };
   // velDistr takes the mass in amu and converts it itself; keep its order.
   auto vel_distr = [&](double v) {
     double mm = P.mu;
@@ -494,7 +489,6 @@ void speed_table(Prepared &P) {
     return std::pow(vv, 5.0) * std::pow((mm / (2.0 * BOLTZMANN_K * T)), 3.0) *
            std::exp(-(mm * std::pow(vv, 2.0)) / (2.0 * BOLTZMANN_K * T));
   };
-  (void)f;
   double v = 0.0;
   const double dv = 1.0E-5;
   double s = vel_distr(v);
